@@ -1,0 +1,182 @@
+"""ctypes binding of libwgm.so (the C ABI declared in include/wgm.h).
+
+The library is built in-tree by __graft_entry__.build() (make -C
+gpu_workgroup_managment_amd/csrc). There is no fallback: if the library is
+missing, importing the package raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwgm.so")
+
+WGM_OK = 0
+ERRORS = {
+    -1: "WGM_ERR_INVALID",
+    -2: "WGM_ERR_NOMEM",
+    -3: "WGM_ERR_HIP",
+    -4: "WGM_ERR_NODEV",
+    -5: "WGM_ERR_TOO_MANY_GROUPS",
+    -6: "WGM_ERR_MISMATCH",
+    -7: "WGM_ERR_TIMEOUT",
+    -8: "WGM_ERR_STATE",
+    -9: "WGM_ERR_BUDGET",
+}
+
+# kernel ids (WGM_K_*)
+K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED = range(9)
+GRAPH_RMAT, GRAPH_GRID = 0, 1
+QUERY_BFS, QUERY_SSSP = 0, 1
+BARRIER_FLAGS, BARRIER_SENSE = 0, 1
+
+
+class WgmError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+class Csr(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("m", ctypes.c_int64),
+        ("row_ptr", ctypes.POINTER(ctypes.c_int32)),
+        ("col", ctypes.POINTER(ctypes.c_int32)),
+        ("wt", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("index", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("query", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("scale", ctypes.c_int32),
+        ("rows", ctypes.c_int32),
+        ("cols", ctypes.c_int32),
+        ("graph_seed", ctypes.c_uint64),
+        ("weight_seed", ctypes.c_uint64),
+    ]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("num_cus", ctypes.c_int32),
+        ("max_wg_size", ctypes.c_int32),
+        ("wavefront_size", ctypes.c_int32),
+        ("wall_clock_hz", ctypes.c_int64),
+        ("total_mem", ctypes.c_int64),
+        ("lds_per_cu", ctypes.c_int32),
+    ]
+
+
+class BarrierResult(ctypes.Structure):
+    _fields_ = [
+        ("participating", ctypes.c_int32),
+        ("barriers", ctypes.c_int32),
+        ("epoch_errors", ctypes.c_int32),
+        ("kernel_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("ns_per_barrier", ctypes.c_double),
+    ]
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [
+        ("wg_size", ctypes.c_int32),
+        ("num_wgs", ctypes.c_int32),
+        ("handshake", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("participating", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("edges_scanned", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("host_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_P = ctypes.POINTER
+
+# name -> (restype, argtypes); every symbol here is declared in include/wgm.h
+SIGNATURES = {
+    "wgm_splitmix64_next": (_u64, [_P(_u64)]),
+    "wgm_gen_er": (_i32, [_i32, _i64, _u64, _P(Csr)]),
+    "wgm_gen_rmat": (_i32, [_i32, _i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _i32, _P(Csr)]),
+    "wgm_gen_grid": (_i32, [_i32, _i32, _P(Csr)]),
+    "wgm_gen_weights": (_i32, [_P(Csr), _i32, _i32, _u64]),
+    "wgm_gen_priorities": (_i32, [_P(Csr), _u64, _vp]),
+    "wgm_csr_free": (None, [_P(Csr)]),
+    "wgm_free": (None, [_vp]),
+    "wgm_batch_spec": (_i32, [_i32, _u64, _i32, _i32, _P(BatchItem)]),
+    "wgm_batch_generate": (_i32, [_i32, _P(BatchItem), _P(Csr), _i32]),
+    "wgm_create": (_i32, [_i32, _P(_vp)]),
+    "wgm_destroy": (_i32, [_vp]),
+    "wgm_last_error": (ctypes.c_char_p, [_vp]),
+    "wgm_strerror": (ctypes.c_char_p, [_i32]),
+    "wgm_get_device_info": (_i32, [_vp, _P(DeviceInfo)]),
+    "wgm_occupancy_query_hip": (_i32, [_vp, _i32, _i32, _P(_i32), _P(_i32)]),
+    "wgm_discover_occupancy": (_i32, [_vp, _i32, _i32, _i32, _P(_i32)]),
+    "wgm_barrier_bench": (_i32, [_vp, _i32, _i32, _i32, _i32, _P(BarrierResult)]),
+    "wgm_graph_create": (_i32, [_vp, _P(Csr), _vp, _P(_vp)]),
+    "wgm_graph_destroy": (_i32, [_vp]),
+    "wgm_graph_bytes": (_i64, [_vp]),
+    "wgm_bfs": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_sssp": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_color": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_mis": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_pagerank": (_i32, [_vp, _vp, ctypes.c_float, _i32, _P(Launch), _vp, _P(RunStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libwgm.so (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, rt=None):
+    if rc != WGM_OK:
+        msg = lib().wgm_last_error(rt)
+        raise WgmError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a):
+    """Raw address of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))

@@ -1,0 +1,424 @@
+// apps.hip -- device-resident graphs and the standalone persistent apps.
+//
+// Plays the role of the reference's persistent-app plugins plus run_persistent:
+// init_persistent_app_for_occupancy uploads the graph, reset_persistent_task
+// re-initialises state, the driver launches the standalone kernel, waits on the
+// handshake and check_persistent_task reads the result back
+// (src/lonestar/graph_apps/bfs/bfs.h:209-370, src/pannotia/graph_apps/color/color.h:41-195,
+// src/pannotia/graph_apps/mis/mis.h:40-202, src/host_skeleton/skel.cpp:250-385).
+// Unlike the reference, a reset never re-uploads the graph: graph arrays stay
+// resident and only the per-run state is re-initialised on the device.
+
+#include "apps_internal.hpp"
+
+#include <cstring>
+
+using namespace wgm;
+
+namespace wgm {
+
+// ====================================================== standalone kernels
+// Shape of each (standalone.cl): discovery; host handshake; barrier; the
+// application loop with global barriers; retire.
+
+template <bool SSSP>
+__global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDev s, int* iters_out) {
+  extern __shared__ int lds[];
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0 || c.im_exit) return;
+  standalone_handshake(c, id, n);
+  global_barrier_disc(c, id, n);
+  clock_start(c, id);
+  int level = 0;
+  while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
+    frontier_level<SSSP>(g, s, level, id, n, lds);
+    global_barrier_disc(c, id, n);
+    level++;
+  }
+  clock_end(c, id);
+  if (id == 0 && threadIdx.x == 0) *iters_out = level;
+  standalone_retire(c);
+}
+
+__global__ void __launch_bounds__(1024) k_color(Ctl c, GraphDev g, int32_t* color, int* iters_out) {
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0 || c.im_exit) return;
+  standalone_handshake(c, id, n);
+  global_barrier_disc(c, id, n);
+  clock_start(c, id);
+  int lo, hi;
+  block_range(g.n, id, n, &lo, &hi);
+  int* flags = c.lines;
+  int round = 1;
+  for (;; round++) {
+    const bool any = color_round(g, color, round, lo, hi);
+    if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
+    if (id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
+    global_barrier_disc(c, id, n);
+    if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
+  }
+  clock_end(c, id);
+  if (id == 0 && threadIdx.x == 0) *iters_out = round;
+  standalone_retire(c);
+}
+
+__global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, int32_t* s_out, int* iters_out) {
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0 || c.im_exit) return;
+  standalone_handshake(c, id, n);
+  global_barrier_disc(c, id, n);
+  clock_start(c, id);
+  int lo, hi;
+  block_range(g.n, id, n, &lo, &hi);
+  int* flags = c.lines;
+  int round = 1;
+  for (;; round++) {
+    const bool any = mis_phase_a(g, st, s_out, round, lo, hi);
+    if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
+    if (id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
+    global_barrier_disc(c, id, n);
+    if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
+    mis_phase_b(g, st, round, lo, hi);
+    global_barrier_disc(c, id, n);
+  }
+  clock_end(c, id);
+  if (id == 0 && threadIdx.x == 0) *iters_out = round;
+  standalone_retire(c);
+}
+
+__global__ void __launch_bounds__(1024) k_pagerank(Ctl c, GraphDev g, float* c0, float* c1, float* rank,
+                                                   float damping, float base, float r0, int iters,
+                                                   int* iters_out) {
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0 || c.im_exit) return;
+  standalone_handshake(c, id, n);
+  global_barrier_disc(c, id, n);
+  clock_start(c, id);
+  int lo, hi;
+  block_range(g.n, id, n, &lo, &hi);
+  pagerank_init(g, c0, r0, lo, hi);
+  global_barrier_disc(c, id, n);
+  for (int it = 0; it < iters; it++) {
+    const float* cin = (it & 1) ? c1 : c0;
+    float* cout = (it & 1) ? c0 : c1;
+    pagerank_iter(g, cin, cout, rank, damping, base, it == iters - 1, lo, hi);
+    global_barrier_disc(c, id, n);
+  }
+  clock_end(c, id);
+  if (id == 0 && threadIdx.x == 0) *iters_out = iters;
+  standalone_retire(c);
+}
+
+// Sum of out-degrees of reached vertices (dist != INF): the TEPS numerator.
+__global__ void k_reached_edges(GraphDev g, const uint32_t* dist, unsigned long long* out) {
+  unsigned long long local = 0;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += gridDim.x * blockDim.x)
+    if (dist[v] != kInf) local += (unsigned long long)(g.row_ptr[v + 1] - g.row_ptr[v]);
+  for (int d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, local);
+}
+
+
+__global__ void k_frontier_init(FrontierDev s, int n, int src) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  for (int v = t; v < n; v += stride) {
+    s.dist[v] = (v == src) ? 0u : kInf;
+    if (s.stamp) s.stamp[v] = 0u;
+  }
+  if (t == 0) {
+    s.q0[0] = src;
+    s.counts[0] = 1;
+    s.counts[kLine] = 0;
+    s.counts[2 * kLine] = 0;
+  }
+}
+
+__global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace wgm
+
+
+#define fail wgm_fail
+#define HIPCHK WGM_HIPCHK
+
+namespace {
+
+template <typename T>
+int dev_alloc(wgm_runtime* rt, T** p, size_t count) {
+  if (*p) return WGM_OK;
+  hipError_t e = hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return wgm_hip_fail(rt, e, "hipMalloc(graph state)");
+  }
+  return WGM_OK;
+}
+
+int launch_prologue(wgm_runtime* rt, int kernel, const wgm_launch* lc, int* grid, int* wg) {
+  wgm_launch def{256, 0, 0, 0};
+  const wgm_launch& l = lc ? *lc : def;
+  *wg = l.wg_size > 0 ? l.wg_size : 256;
+  int rc = wgm_resolve_grid(rt, kernel, *wg, l.num_wgs, grid);
+  if (rc) return rc;
+  rc = wgm_reset_ctl(rt, 0, l.handshake ? 1 : 0);
+  if (rc) return rc;
+  HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+  return WGM_OK;
+}
+
+int launch_epilogue(wgm_runtime* rt, const wgm_launch* lc, wgm_graph* g, wgm_run_stats* st) {
+  HIPCHK(rt, hipGetLastError());
+  int rc = WGM_OK;
+  double host_ms = 0.0;
+  if (lc && lc->handshake) rc = wgm_host_handshake(rt, &host_ms);
+  wgm_run_stats local{};
+  int rc2 = wgm_finish_launch(rt, &local);
+  if (rc == WGM_OK) rc = rc2;
+  if (rc) return rc;
+  int iters = 0;
+  HIPCHK(rt, hipMemcpy(&iters, g->iters, sizeof(int), hipMemcpyDeviceToHost));
+  local.iterations = iters;
+  local.host_ms = host_ms;
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+}  // namespace
+
+GraphDev wgm_graph_view(const wgm_graph* g) {
+  GraphDev v{};
+  v.row_ptr = g->row_ptr;
+  v.col = g->col;
+  v.wt = g->wt;
+  v.prio = g->prio;
+  v.n = g->n;
+  v.m = (int32_t)g->m;
+  return v;
+}
+
+int wgm_graph_frontier_state(wgm_graph* g, bool sssp, FrontierDev* s) {
+  wgm_runtime* rt = g->rt;
+  int rc = dev_alloc(rt, &g->dist, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->q0, g->n + 1);
+  if (!rc) rc = dev_alloc(rt, &g->q1, g->n + 1);
+  if (!rc) rc = dev_alloc(rt, &g->counts, 3 * kLine);
+  if (!rc && sssp) rc = dev_alloc(rt, &g->stamp, g->n);
+  if (rc) return rc;
+  s->dist = g->dist;
+  s->stamp = sssp ? g->stamp : nullptr;
+  s->q0 = g->q0;
+  s->q1 = g->q1;
+  s->counts = g->counts;
+  return WGM_OK;
+}
+
+int wgm_graph_reached_edges(wgm_graph* g, int64_t* out) {
+  wgm_runtime* rt = g->rt;
+  unsigned long long* d = nullptr;
+  HIPCHK(rt, hipMalloc(&d, sizeof(unsigned long long)));
+  (void)hipMemsetAsync(d, 0, sizeof(unsigned long long), rt->stream);
+  hipLaunchKernelGGL(k_reached_edges, dim3(256), dim3(256), 0, rt->stream, wgm_graph_view(g), g->dist, d);
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, rt->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return wgm_hip_fail(rt, e, "reached edges");
+  *out = (int64_t)h;
+  return WGM_OK;
+}
+
+bool wgm_app_kernel_info(int kernel, KernelInfo* out) {
+  switch (kernel) {
+    case WGM_K_BFS: *out = KernelInfo{(const void*)&k_frontier<false>, 1}; return true;
+    case WGM_K_SSSP: *out = KernelInfo{(const void*)&k_frontier<true>, 1}; return true;
+    case WGM_K_COLOR: *out = KernelInfo{(const void*)&k_color, 0}; return true;
+    case WGM_K_MIS: *out = KernelInfo{(const void*)&k_mis, 0}; return true;
+    case WGM_K_PAGERANK: *out = KernelInfo{(const void*)&k_pagerank, 0}; return true;
+    default: return false;
+  }
+}
+
+// Discovery probe with the app kernel itself (im_exit set: every workgroup
+// returns right after discovery and never reads its other arguments).
+int wgm_app_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
+  const size_t lds = wgm_kernel_lds(kernel, wg_size);
+  switch (kernel) {
+    case WGM_K_BFS:
+      hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg_size), lds, rt->stream, rt->ctl, GraphDev{},
+                         FrontierDev{}, (int*)nullptr);
+      return WGM_OK;
+    case WGM_K_SSSP:
+      hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg_size), lds, rt->stream, rt->ctl, GraphDev{},
+                         FrontierDev{}, (int*)nullptr);
+      return WGM_OK;
+    case WGM_K_COLOR:
+      hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (int32_t*)nullptr,
+                         (int*)nullptr);
+      return WGM_OK;
+    case WGM_K_MIS:
+      hipLaunchKernelGGL(k_mis, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (int32_t*)nullptr,
+                         (int32_t*)nullptr, (int*)nullptr);
+      return WGM_OK;
+    case WGM_K_PAGERANK:
+      hipLaunchKernelGGL(k_pagerank, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (float*)nullptr,
+                         (float*)nullptr, (float*)nullptr, 0.f, 0.f, 0.f, 0, (int*)nullptr);
+      return WGM_OK;
+    default:
+      return WGM_ERR_INVALID;
+  }
+}
+
+extern "C" {
+
+int wgm_graph_create(wgm_runtime* rt, const wgm_csr* h, const uint32_t* prio, wgm_graph** out) {
+  if (!rt || !h || !out) return fail(rt, WGM_ERR_INVALID, "NULL argument");
+  if (h->n < 1 || h->m < 0 || h->m > INT32_MAX || !h->row_ptr || (h->m > 0 && !h->col))
+    return fail(rt, WGM_ERR_INVALID, "bad CSR shape n=%d m=%lld", h->n, (long long)h->m);
+  if (h->row_ptr[0] != 0 || h->row_ptr[h->n] != h->m)
+    return fail(rt, WGM_ERR_INVALID, "row_ptr must start at 0 and end at m");
+  for (int32_t v = 0; v < h->n; v++)
+    if (h->row_ptr[v + 1] < h->row_ptr[v]) return fail(rt, WGM_ERR_INVALID, "row_ptr not monotone at %d", v);
+  for (int64_t e = 0; e < h->m; e++)
+    if (h->col[e] < 0 || h->col[e] >= h->n) return fail(rt, WGM_ERR_INVALID, "col[%lld] out of range", (long long)e);
+  wgm_graph* g = new wgm_graph();
+  g->rt = rt;
+  g->n = h->n;
+  g->m = h->m;
+  auto bail = [&](int rc) { wgm_graph_destroy(g); return rc; };
+  int rc = dev_alloc(rt, &g->row_ptr, (size_t)h->n + 1);
+  if (!rc) rc = dev_alloc(rt, &g->col, (size_t)h->m);
+  if (!rc && h->wt) rc = dev_alloc(rt, &g->wt, (size_t)h->m);
+  if (!rc && prio) rc = dev_alloc(rt, &g->prio, (size_t)h->n);
+  if (!rc) rc = dev_alloc(rt, &g->iters, 1);
+  if (rc) return bail(rc);
+  hipError_t e = hipMemcpy(g->row_ptr, h->row_ptr, sizeof(int32_t) * ((size_t)h->n + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && h->m) e = hipMemcpy(g->col, h->col, sizeof(int32_t) * (size_t)h->m, hipMemcpyHostToDevice);
+  if (e == hipSuccess && h->wt && h->m) e = hipMemcpy(g->wt, h->wt, sizeof(int32_t) * (size_t)h->m, hipMemcpyHostToDevice);
+  if (e == hipSuccess && prio) e = hipMemcpy(g->prio, prio, sizeof(uint32_t) * (size_t)h->n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "graph upload"));
+  *out = g;
+  return WGM_OK;
+}
+
+int wgm_graph_destroy(wgm_graph* g) {
+  if (!g) return WGM_OK;
+  void* ptrs[] = {g->row_ptr, g->col, g->wt, g->prio, g->dist, g->stamp, g->q0, g->q1, g->counts,
+                  g->i32a, g->i32b, g->f0, g->f1, g->f2, g->iters};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete g;
+  return WGM_OK;
+}
+
+int64_t wgm_graph_bytes(const wgm_graph* g) {
+  if (!g) return 0;
+  int64_t b = 4 * ((int64_t)g->n + 1) + 4 * g->m;
+  if (g->wt) b += 4 * g->m;
+  if (g->prio) b += 4 * (int64_t)g->n;
+  return b;
+}
+
+static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, uint32_t* dist_out,
+                        wgm_run_stats* st, bool sssp) {
+  if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
+  if (src < 0 || src >= g->n) return fail(rt, WGM_ERR_INVALID, "source %d out of range", src);
+  if (sssp && !g->wt) return fail(rt, WGM_ERR_INVALID, "SSSP needs edge weights");
+  FrontierDev s{};
+  int rc = wgm_graph_frontier_state(g, sssp, &s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_frontier_init, dim3(256), dim3(256), 0, rt->stream, s, g->n, src);
+  const int kernel = sssp ? WGM_K_SSSP : WGM_K_BFS;
+  int grid = 0, wg = 0;
+  rc = launch_prologue(rt, kernel, lc, &grid, &wg);
+  if (rc) return rc;
+  const size_t lds = frontier_lds_bytes(wg);
+  if (sssp)
+    hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, g->iters);
+  else
+    hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, g->iters);
+  wgm_run_stats local{};
+  rc = launch_epilogue(rt, lc, g, &local);
+  if (rc) return rc;
+  rc = wgm_graph_reached_edges(g, &local.edges_scanned);
+  if (rc) return rc;
+  if (dist_out) HIPCHK(rt, hipMemcpy(dist_out, g->dist, sizeof(uint32_t) * (size_t)g->n, hipMemcpyDeviceToHost));
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+int wgm_bfs(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, uint32_t* dist_out,
+            wgm_run_stats* st) {
+  return run_frontier(rt, g, src, lc, dist_out, st, false);
+}
+
+int wgm_sssp(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, uint32_t* dist_out,
+             wgm_run_stats* st) {
+  return run_frontier(rt, g, src, lc, dist_out, st, true);
+}
+
+int wgm_color(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* color_out, wgm_run_stats* st) {
+  if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
+  if (!g->prio) return fail(rt, WGM_ERR_INVALID, "colouring needs vertex priorities");
+  int rc = dev_alloc(rt, &g->i32a, g->n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, -1);
+  int grid = 0, wg = 0;
+  rc = launch_prologue(rt, WGM_K_COLOR, lc, &grid, &wg);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->iters);
+  rc = launch_epilogue(rt, lc, g, st);
+  if (rc) return rc;
+  if (color_out) HIPCHK(rt, hipMemcpy(color_out, g->i32a, sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost));
+  return WGM_OK;
+}
+
+int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out, wgm_run_stats* st) {
+  if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
+  if (!g->prio) return fail(rt, WGM_ERR_INVALID, "MIS needs vertex priorities");
+  int rc = dev_alloc(rt, &g->i32a, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->i32b, g->n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, -1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32b, g->n, 0);
+  int grid = 0, wg = 0;
+  rc = launch_prologue(rt, WGM_K_MIS, lc, &grid, &wg);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mis, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->i32b,
+                     g->iters);
+  rc = launch_epilogue(rt, lc, g, st);
+  if (rc) return rc;
+  if (s_out) HIPCHK(rt, hipMemcpy(s_out, g->i32b, sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost));
+  return WGM_OK;
+}
+
+int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations, const wgm_launch* lc,
+                 float* rank_out, wgm_run_stats* st) {
+  if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
+  if (iterations < 1 || !(damping >= 0.0f && damping <= 1.0f))
+    return fail(rt, WGM_ERR_INVALID, "need iterations >= 1 and damping in [0, 1]");
+  int rc = dev_alloc(rt, &g->f0, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->f1, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->f2, g->n);
+  if (rc) return rc;
+  const float nf = (float)g->n;
+  const float base = (1.0f - damping) / nf;
+  const float r0 = 1.0f / nf;
+  int grid = 0, wg = 0;
+  rc = launch_prologue(rt, WGM_K_PAGERANK, lc, &grid, &wg);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pagerank, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->f0, g->f1, g->f2,
+                     damping, base, r0, iterations, g->iters);
+  rc = launch_epilogue(rt, lc, g, st);
+  if (rc) return rc;
+  if (rank_out) HIPCHK(rt, hipMemcpy(rank_out, g->f2, sizeof(float) * (size_t)g->n, hipMemcpyDeviceToHost));
+  return WGM_OK;
+}
+
+}  // extern "C"

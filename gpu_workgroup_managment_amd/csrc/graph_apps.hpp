@@ -1,0 +1,228 @@
+// graph_apps.hpp -- persistent graph kernels for gfx950 on the wgm runtime.
+//
+// Frontier BFS / SSSP (Lonestar drelax2: src/lonestar/graph_apps/bfs/device/bfs.cl:104-302,
+// src/lonestar/graph_apps/sssp/device/sssp.cl:104-318), topology-driven
+// colouring and MIS (Pannotia color_combined / mis_combined:
+// src/pannotia/graph_apps/color/device/color.cl:3-97,
+// src/pannotia/graph_apps/mis/device/mis.cl:5-125) and pull PageRank.
+//
+// The per-level / per-round bodies take (rank, nteam): the workgroup's index
+// inside the set of workgroups that share the graph, and that set's size. The
+// standalone kernels pass the discovered id and count; the batched kernel
+// passes a team; the mega-kernel passes the resizable persistent ids.
+#pragma once
+
+#include "wgm_device.hpp"
+
+namespace wgm {
+
+constexpr uint32_t kInf = 1000000000u;  // MYINFINITY (src/lonestar/graph_apps/bfs/device/bfs.cl:33)
+
+struct GraphDev {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* wt;
+  const uint32_t* prio;
+  int32_t n;
+  int32_t m;
+};
+
+// Frontier state. Level L's frontier is q[L & 1] with its length in
+// counts[(L % 3) * kLine]; three counters let one barrier end a level: during
+// level L the workgroups read counter L%3, push into (L+1)%3 and rank 0 zeroes
+// (L+2)%3, whose last readers finished before the previous barrier.
+struct FrontierDev {
+  uint32_t* dist;
+  uint32_t* stamp;  // SSSP: last level v was queued for (dedups pushes)
+  int32_t* q0;
+  int32_t* q1;
+  int* counts;      // 3 lines
+};
+
+// Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17]
+__host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 17) * 4; }
+
+// One level of frontier relaxation over q[level & 1]. Chunks of bd frontier
+// vertices go to team workgroups round-robin; inside a chunk the workgroup
+// scans the degrees and spreads the chunk's edges evenly over its threads,
+// each finding its source vertex by binary search in LDS, so a high-degree
+// vertex does not serialise one thread. BFS claims a vertex with one CAS
+// (INF -> level+1) and pushes it exactly once; SSSP lowers dist with
+// atomic min and pushes a vertex at most once per level through `stamp`.
+// Reference: processnode2 / processedge2 / wl_push_1item
+// (src/lonestar/graph_apps/bfs/device/bfs.cl:82-212).
+template <bool SSSP>
+__device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
+                                               int rank, int nteam, int* lds) {
+  const int bd = blockDim.x, tid = threadIdx.x;
+  int* s_end = lds;
+  int* s_beg = lds + bd;
+  uint32_t* s_du = (uint32_t*)(lds + 2 * bd);
+  int* s_wave = lds + 3 * bd;
+  const int in_n = ld_relaxed(&s.counts[(level % 3) * kLine]);
+  const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
+  int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+  int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
+  const uint32_t next = (uint32_t)level + 1u;
+
+  for (int base = rank * bd; base < in_n; base += nteam * bd) {
+    const int i = base + tid;
+    int beg = 0, deg = 0;
+    uint32_t du = 0;
+    if (i < in_n) {
+      const int v = in_q[i];
+      beg = g.row_ptr[v];
+      deg = g.row_ptr[v + 1] - beg;
+      if (SSSP) du = s.dist[v];
+    }
+    int total;
+    const int excl = wg_exclusive_scan(deg, s_wave, &total);
+    s_end[tid] = excl + deg;
+    s_beg[tid] = beg - excl;
+    if (SSSP) s_du[tid] = du;
+    __syncthreads();
+    for (int eb = 0; eb < total; eb += bd) {
+      const int e = eb + tid;
+      bool want = false;
+      int u = 0;
+      if (e < total) {
+        int lo = 0, hi = bd - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_end[mid] > e) hi = mid; else lo = mid + 1;
+        }
+        const int idx = s_beg[lo] + e;
+        u = g.col[idx];
+        if (!SSSP) {
+          if (s.dist[u] == kInf) {
+            uint32_t expect = kInf;
+            want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED,
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        } else {
+          const uint32_t alt = s_du[lo] + (uint32_t)g.wt[idx];
+          if (alt < s.dist[u]) {
+            const uint32_t old =
+                __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (alt < old)
+              want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < next;
+          }
+        }
+      }
+      wave_push(want, u, out_q, out_cnt);
+    }
+    __syncthreads();
+  }
+  if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
+}
+
+// ----------------------------------------------------------- colouring
+// One barrier per round instead of the reference's two: during round c a
+// neighbour is "uncoloured at the start of the round" iff its colour is -1 or
+// c, so the max-priority test and the colour write share one pass with the
+// same outcome as colour1 -> barrier -> colour2. The degree-1 quirk
+// (`start != end - 1`, src/pannotia/graph_apps/color/device/color.cl:46) is kept:
+// a vertex with exactly one neighbour ignores it. The stop flag of round c is
+// line c % 3 of `flags`.
+__device__ __forceinline__ bool color_round(const GraphDev& g, int32_t* color, int round, int lo, int hi) {
+  bool any = false;
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (color[v] != -1) continue;
+    const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
+    long long mx = -1;
+    if (end - beg != 1) {
+      for (int e = beg; e < end; e++) {
+        const int u = g.col[e];
+        const int cu = color[u];
+        if (cu == -1 || cu == round) {
+          any = true;
+          const long long pu = (long long)g.prio[u];
+          mx = pu > mx ? pu : mx;
+        }
+      }
+    }
+    if ((long long)g.prio[v] > mx) color[v] = round;
+  }
+  return any;
+}
+
+// ------------------------------------------------------------------ MIS
+// State per vertex: -1 undecided, 2r joined in round r, 2r+1 removed in
+// round r. Phase A (= mis1 + the join half of mis2): an undecided vertex joins
+// if its priority is below every neighbour undecided at the round start
+// (state -1 or 2r). Phase B (= the neighbour half of mis2 + mis3): an
+// undecided vertex with a neighbour that joined this round is removed.
+// Two barriers per round instead of three; the sets are those of
+// src/pannotia/graph_apps/mis/device/mis.cl:21-124.
+__device__ __forceinline__ bool mis_phase_a(const GraphDev& g, int32_t* st, int32_t* s_out, int round,
+                                            int lo, int hi) {
+  bool any = false;
+  const int joined = 2 * round;
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (st[v] != -1) continue;
+    any = true;
+    long long mn = 0x100000000LL;  // above every uint32 priority
+    for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
+      const int u = g.col[e];
+      const int su = st[u];
+      if (su == -1 || su == joined) {
+        const long long pu = (long long)g.prio[u];
+        mn = pu < mn ? pu : mn;
+      }
+    }
+    if ((long long)g.prio[v] < mn) {
+      st[v] = joined;
+      s_out[v] = 2;
+    }
+  }
+  return any;
+}
+
+__device__ __forceinline__ void mis_phase_b(const GraphDev& g, int32_t* st, int round, int lo, int hi) {
+  const int joined = 2 * round;
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (st[v] != -1) continue;
+    for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
+      if (st[g.col[e]] == joined) {
+        st[v] = joined + 1;
+        break;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ PageRank
+// Pull iteration r'(v) = base + d * sum_{u in N(v)} r(u)/deg(u), FP32, the sum
+// taken in CSR order by one thread, so the result is bitwise reproducible.
+// contrib holds r(u)/deg(u) (0 for deg 0).
+__device__ __forceinline__ void pagerank_init(const GraphDev& g, float* contrib, float r0, int lo, int hi) {
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    const int deg = g.row_ptr[v + 1] - g.row_ptr[v];
+    contrib[v] = deg ? r0 / (float)deg : 0.0f;
+  }
+}
+
+__device__ __forceinline__ void pagerank_iter(const GraphDev& g, const float* cin, float* cout, float* rank,
+                                              float damping, float base, bool last, int lo, int hi) {
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
+    float sum = 0.0f;
+    for (int e = beg; e < end; e++) sum = sum + cin[g.col[e]];
+    const float r = __builtin_fmaf(damping, sum, base);
+    if (last) rank[v] = r;
+    else cout[v] = (end > beg) ? r / (float)(end - beg) : 0.0f;
+  }
+}
+
+// Contiguous vertex block of workgroup `rank` among `nteam` (keeps grid
+// neighbours in one workgroup and so in one XCD's L2).
+__device__ __forceinline__ void block_range(int n, int rank, int nteam, int* lo, int* hi) {
+  const long long per = ((long long)n + nteam - 1) / nteam;
+  long long a = per * rank, b = a + per;
+  if (a > n) a = n;
+  if (b > n) b = n;
+  *lo = (int)a;
+  *hi = (int)b;
+}
+
+}  // namespace wgm

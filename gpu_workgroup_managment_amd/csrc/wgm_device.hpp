@@ -1,0 +1,321 @@
+// wgm_device.hpp -- device-side cooperative workgroup runtime for gfx950.
+//
+// Occupancy discovery, inter-workgroup barriers, per-task virtual workgroup
+// ids, and the resizing scheduler's offer_kill / offer_fork, written for a
+// wave64 CDNA4 part with eight XCDs. Device-internal flags are agent-scope
+// atomics in HBM, one 128-byte line per workgroup; cells the host watches are
+// system-scope atomics in coherent host memory.
+//
+// Reference counterparts (behaviour kept, mechanism redesigned):
+//   discovery      src/scheduler_rt/rt_device/discovery.cl:26-72
+//   flag barrier   src/scheduler_rt/rt_device/iw_barrier.cl:63-109
+//   resize barrier src/scheduler_rt/rt_device/iw_barrier.cl:165-227
+//   sense barrier  src/scheduler_rt/rt_device/iw_barrier.cl:315-351
+//   kernel ctx     src/scheduler_rt/rt_device/kernel_ctx.cl:6-28
+//   scheduler      src/scheduler_rt/rt_device/cl_scheduler.cl:8-171,
+//                  src/scheduler_rt/rt_device/scheduler_1.cl:10-138
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wgm {
+
+constexpr int kLine = 32;  // ints per 128-byte line
+
+// ------------------------------------------------------------------ atomics
+__device__ __forceinline__ int ld_relaxed(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_acquire(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int add_relaxed(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int add_acq_rel(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int or_acq_rel(int* p, int v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int xchg_acquire(int* p, int v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+// System scope: cells shared with the host through coherent host memory.
+__device__ __forceinline__ int sys_ld_acquire(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int sys_ld_relaxed(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_st_release(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int sys_add_release(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_st_u64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void spin_pause() { __builtin_amdgcn_s_sleep(1); }
+
+// ---------------------------------------------------- shared control layout
+// Host-side set-up (runtime.hip) allocates and resets these; kernels receive
+// the struct by value. Every int that is spun on lives in its own line.
+struct Ctl {
+  // discovery (a1-a5)
+  int* poll;         // bit 31: poll closed; low bits: joins
+  int* count;        // participant count, -1 until the first closer publishes it
+  int* p_group_ids;  // hardware workgroup id -> discovered id, -1 if not participating
+  // barrier (a6-a11)
+  int* flags;        // flag of discovered workgroup i at flags[i * kLine]
+  int* bar_groups;   // resizable barrier group count (bar->num_groups)
+  int* bar_to_kill;  // query-barrier kill count (bar->to_kill)
+  int* sense_counter;
+  int* sense_flag;
+  // per-launch scratch lines (level counters, stop flags, ...)
+  int* lines;        // kNumLines lines
+  unsigned long long* clock;  // [0] start tick, [1] end tick
+  // host-watched cells (coherent host memory)
+  int* h_scheduler_flag;
+  int* h_persistent_flag;
+  int* h_participating;
+  int max_groups;
+  int im_exit;
+  unsigned int window_ticks;  // discovery quiescence window (device clock)
+  int handshake;              // standalone host handshake on/off
+};
+constexpr int kNumLines = 16;
+
+// scheduler flag values (src/scheduler_rt/rt_common/cl_scheduler.h:15-21)
+constexpr int DEVICE_SCHEDULER_INIT = 0;
+constexpr int DEVICE_WAITING = 1;
+constexpr int DEVICE_TO_QUIT = 2;
+constexpr int DEVICE_TO_TASK = 3;
+constexpr int DEVICE_GOT_GROUPS = 4;
+constexpr int DEVICE_TO_EXECUTE = 5;
+constexpr int DEVICE_TO_PERSISTENT_TASK = 6;
+// task mailbox values (src/scheduler_rt/rt_common/constants.h:4-8)
+constexpr int TASK_UNINIT = -2;
+constexpr int TASK_WAIT = -1;
+constexpr int TASK_QUIT = 0;
+constexpr int TASK_MULT = 1;
+constexpr int TASK_PERSIST = 2;
+
+constexpr int kPollClosed = 0x40000000;
+constexpr int kPollMask = 0x3fffffff;
+
+// --------------------------------------------------------------- discovery
+// One fetch_add joins the poll; the join order gives dense ids 0..count-1.
+// The poll closes with one fetch_or once every launched workgroup has joined
+// or the quiescence window has passed since this workgroup joined; the first
+// closer publishes the count. A workgroup whose join sees the poll closed was
+// not co-resident with the participants and leaves. Replaces the ticket-lock
+// poll with its 12 serialised lock rounds per workgroup
+// (src/scheduler_rt/rt_device/discovery.cl:26-57).
+// Returns the discovered id, or -1 for a non-participant. *count_out gets the
+// participant count (valid for participants).
+__device__ __forceinline__ int discovery_protocol(const Ctl& c, int* count_out) {
+  __shared__ int s_disc[2];
+  if (threadIdx.x == 0) {
+    int id = -1, count = 0;
+    int old = add_relaxed(c.poll, 1);
+    // Joins beyond the flag table are turned away like late arrivals.
+    if (!(old & kPollClosed) && (old & kPollMask) < c.max_groups) {
+      id = old & kPollMask;
+      c.p_group_ids[blockIdx.x] = id;
+      const unsigned long long t0 = wall_clock64();
+      const int launched = (int)gridDim.x;
+      while (true) {
+        int v = ld_relaxed(c.poll);
+        if ((v & kPollClosed) || (v & kPollMask) >= launched) break;
+        if (wall_clock64() - t0 > c.window_ticks) break;
+        spin_pause();
+      }
+      int before = or_acq_rel(c.poll, kPollClosed);
+      if (!(before & kPollClosed)) {
+        count = before & kPollMask;
+        if (count > c.max_groups) count = c.max_groups;
+        st_release(c.count, count);
+      } else {
+        while ((count = ld_acquire(c.count)) < 0) spin_pause();
+      }
+    }
+    s_disc[0] = id;
+    s_disc[1] = count;
+  }
+  __syncthreads();
+  *count_out = s_disc[1];
+  return s_disc[0];
+}
+
+// ------------------------------------------------------------ flag barrier
+// Master/slave barrier over discovered ids 0..n-1 (global_barrier_disc).
+// Slaves publish arrival in their own line and spin there; the master polls
+// up to eight peers per thread with all loads in flight at once, then
+// releases every slave. One agent-scope release fence before each batch of
+// relaxed flag stores and one acquire fence after each spin.
+__device__ __forceinline__ void master_collect(int* flags, int lo, int hi) {
+  // wait for flags[p] == 1 for p in [lo, hi)
+  const int tid = threadIdx.x, bd = blockDim.x;
+  if (hi - lo <= 8 * bd) {
+    unsigned pending = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (lo + tid + k * bd < hi) pending |= 1u << k;
+    while (pending) {
+      int v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        v[k] = ((pending >> k) & 1u) ? ld_relaxed(&flags[(lo + tid + k * bd) * kLine]) : 1;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (v[k] != 0) pending &= ~(1u << k);
+    }
+  } else {
+    for (int p = lo + tid; p < hi; p += bd)
+      while (ld_relaxed(&flags[p * kLine]) == 0) {
+      }
+  }
+}
+
+__device__ __forceinline__ void master_release(int* flags, int lo, int hi) {
+  fence_release();
+  for (int p = lo + (int)threadIdx.x; p < hi; p += blockDim.x) st_relaxed(&flags[p * kLine], 0);
+}
+
+__device__ __forceinline__ void slave_arrive_wait(int* flags, int id) {
+  int* f = &flags[id * kLine];
+  fence_release();
+  st_relaxed(f, 1);
+  while (ld_relaxed(f) != 0) {
+  }
+  fence_acquire();
+}
+
+__device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n) {
+  if (n <= 1) {
+    __syncthreads();
+    return;
+  }
+  if (id == 0) {
+    master_collect(c.flags, 1, n);
+    fence_acquire();
+    __syncthreads();
+    master_release(c.flags, 1, n);
+  } else {
+    __syncthreads();
+    if (threadIdx.x == 0) slave_arrive_wait(c.flags, id);
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------- sense-reversal barrier
+// Counter + sense barrier whose group count is re-read on every spin
+// (global_barrier_robust_to_resizing); `*groups` may change between calls.
+__device__ __forceinline__ void global_barrier_sense(int* counter, int* sense_word, const int* groups,
+                                                     int* local_sense) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int s = !(*local_sense);
+    *local_sense = s;
+    if (add_acq_rel(counter, 1) == 0) {
+      while (true) {
+        const int arrived = ld_acquire(counter);
+        const int n = ld_relaxed(groups);
+        if (arrived == n) {
+          st_relaxed(counter, 0);
+          st_release(sense_word, s);
+          break;
+        }
+      }
+    } else {
+      while (ld_relaxed(sense_word) != s) {
+      }
+      fence_acquire();
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------- standalone prologue
+// Group 0 publishes persistent_flag = count and DEVICE_WAITING, then waits for
+// DEVICE_TO_PERSISTENT_TASK (src/lonestar/graph_apps/bfs/device/standalone.cl:277-290).
+__device__ __forceinline__ void standalone_handshake(const Ctl& c, int id, int n) {
+  if (c.handshake && id == 0 && threadIdx.x == 0) {
+    *c.h_participating = n;
+    sys_st_release(c.h_persistent_flag, n);
+    sys_st_release(c.h_scheduler_flag, DEVICE_WAITING);
+    while (sys_ld_relaxed(c.h_scheduler_flag) != DEVICE_TO_PERSISTENT_TASK) spin_pause();
+    fence_acquire();
+  }
+}
+
+// Epilogue: each workgroup retires from the persistent task
+// (src/lonestar/graph_apps/bfs/device/standalone.cl:316-319).
+__device__ __forceinline__ void standalone_retire(const Ctl& c) {
+  if (c.handshake && threadIdx.x == 0) sys_add_release(c.h_persistent_flag, -1);
+}
+
+__device__ __forceinline__ void clock_start(const Ctl& c, int id) {
+  if (id == 0 && threadIdx.x == 0) c.clock[0] = wall_clock64();
+}
+__device__ __forceinline__ void clock_end(const Ctl& c, int id) {
+  if (id == 0 && threadIdx.x == 0) c.clock[1] = wall_clock64();
+}
+
+// --------------------------------------------------------- workgroup scans
+// Exclusive prefix sum over the workgroup (blockDim.x a multiple of 64).
+// s_wave must hold 17 ints. Returns the exclusive prefix; *total = sum.
+__device__ __forceinline__ int wg_exclusive_scan(int x, int* s_wave, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < nw ? s_wave[lane] : 0;
+    int winc = w;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      int y = __shfl_up(winc, d, 64);
+      if (lane >= d) winc += y;
+    }
+    if (lane < nw) s_wave[lane] = winc - w;
+    if (lane == nw - 1) s_wave[16] = winc;
+  }
+  __syncthreads();
+  const int excl = inc - x + s_wave[wid];
+  *total = s_wave[16];
+  __syncthreads();
+  return excl;
+}
+
+// Wave-aggregated push: one atomic per wave, lanes write consecutive slots.
+__device__ __forceinline__ void wave_push(bool want, int item, int* queue, int* count) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = add_relaxed(count, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (want) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = item;
+}
+
+}  // namespace wgm

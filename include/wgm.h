@@ -1,0 +1,183 @@
+/*
+ * wgm.h -- C ABI of the MI355X cooperative workgroup runtime (libwgm.so).
+ *
+ * Every entry point returns an int status (WGM_OK or a negative WGM_ERR_*)
+ * and never calls exit(). The reference instead calls check_ocl() -> exit(1)
+ * on any OpenCL error (src/scheduler_rt/rt_host/ggc_ocl.h:11-18) and exits when
+ * too many workgroups are requested (src/scheduler_rt/rt_host/cl_communicator.h:125-130).
+ *
+ * Threading: one wgm_runtime per GPU; calls on one runtime are serialised by
+ * the caller (the reference's CL_Communicator is not re-entrant either,
+ * src/scheduler_rt/rt_host/cl_communicator.h:19-465).
+ */
+#ifndef WGM_H
+#define WGM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define WGM_OK 0
+#define WGM_ERR_INVALID (-1)          /* bad argument or shape */
+#define WGM_ERR_NOMEM (-2)            /* host or device allocation failed */
+#define WGM_ERR_HIP (-3)              /* a HIP call failed; see wgm_last_error */
+#define WGM_ERR_NODEV (-4)            /* no usable GPU */
+#define WGM_ERR_TOO_MANY_GROUPS (-5)  /* more workgroups asked for than discovered */
+#define WGM_ERR_MISMATCH (-6)         /* a result self-check failed */
+#define WGM_ERR_TIMEOUT (-7)          /* a bounded host wait expired */
+#define WGM_ERR_STATE (-8)            /* call not valid in the current protocol state */
+#define WGM_ERR_BUDGET (-9)           /* admission refused: HBM budget exceeded */
+
+/* ------------------------------------------------------- graphs (host CSR) */
+/* 0-based CSR: row_ptr[n+1], col[m] ascending per row, optional wt[m]. */
+typedef struct wgm_csr {
+  int32_t n;
+  int64_t m;
+  int32_t* row_ptr;
+  int32_t* col;
+  int32_t* wt;
+} wgm_csr;
+
+#define WGM_GRAPH_RMAT 0
+#define WGM_GRAPH_GRID 1
+#define WGM_QUERY_BFS 0
+#define WGM_QUERY_SSSP 1
+
+typedef struct wgm_batch_item {
+  int32_t index;
+  int32_t kind;   /* WGM_GRAPH_* */
+  int32_t query;  /* WGM_QUERY_* */
+  int32_t n;      /* vertices after truncation / squaring */
+  int32_t scale;  /* RMAT scale (0 for grids) */
+  int32_t rows, cols;
+  uint64_t graph_seed;
+  uint64_t weight_seed;
+} wgm_batch_item;
+
+/* splitmix64 (Vigna): state += 0x9e3779b97f4a7c15, then the 64-bit mixer. */
+uint64_t wgm_splitmix64_next(uint64_t* state);
+/* G(n, M): the first `pairs` distinct non-loop pairs, symmetrised. */
+int wgm_gen_er(int32_t n, int64_t pairs, uint64_t seed, wgm_csr* out);
+/* RMAT, edge_factor * 2^scale draws, self-loops and duplicates removed,
+ * edges with an endpoint >= truncate_n dropped (truncate_n <= 0: keep all). */
+int wgm_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, int32_t truncate_n, wgm_csr* out);
+/* rows x cols 4-neighbour grid, id = r * cols + c, undirected. */
+int wgm_gen_grid(int32_t rows, int32_t cols, wgm_csr* out);
+/* wt[e] uniform in [lo, hi] in CSR order. Allocates g->wt. */
+int wgm_gen_weights(wgm_csr* g, int32_t lo, int32_t hi, uint64_t seed);
+/* uint32 priorities, no two adjacent vertices equal. */
+int wgm_gen_priorities(const wgm_csr* g, uint64_t seed, uint32_t* prio);
+void wgm_csr_free(wgm_csr* g);
+/* free() for single arrays the library allocated (e.g. wgm_gen_weights' wt). */
+void wgm_free(void* p);
+/* Batched workload description (count items) and parallel generation. */
+int wgm_batch_spec(int32_t count, uint64_t seed, int32_t lo_exp, int32_t hi_exp,
+                   wgm_batch_item* items);
+int wgm_batch_generate(int32_t count, const wgm_batch_item* items, wgm_csr* graphs,
+                       int32_t threads);
+
+/* --------------------------------------------------------------- runtime */
+typedef struct wgm_runtime wgm_runtime;
+
+typedef struct wgm_device_info {
+  char name[64];
+  char arch[32];
+  int32_t num_cus;
+  int32_t max_wg_size;
+  int32_t wavefront_size;
+  int64_t wall_clock_hz;   /* rate of the device constant clock */
+  int64_t total_mem;
+  int32_t lds_per_cu;
+} wgm_device_info;
+
+/* Kernels whose occupancy can be queried / discovered. */
+#define WGM_K_DISCOVERY 0
+#define WGM_K_BARRIER_BENCH 1
+#define WGM_K_BFS 2
+#define WGM_K_SSSP 3
+#define WGM_K_COLOR 4
+#define WGM_K_MIS 5
+#define WGM_K_PAGERANK 6
+#define WGM_K_MEGA 7
+#define WGM_K_BATCHED 8
+#define WGM_K_COUNT 9
+
+int wgm_create(int32_t device, wgm_runtime** out);
+int wgm_destroy(wgm_runtime* rt);
+const char* wgm_last_error(const wgm_runtime* rt);
+const char* wgm_strerror(int32_t status);
+int wgm_get_device_info(wgm_runtime* rt, wgm_device_info* out);
+
+/* hipOccupancyMaxActiveBlocksPerMultiprocessor for `kernel` at wg_size,
+ * and the CU count. Cross-check for discovery (north-star a5). */
+int wgm_occupancy_query_hip(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
+                            int32_t* blocks_per_cu, int32_t* num_cus);
+
+/* Occupancy discovery probe: launch `probe_groups` workgroups of `kernel`'s
+ * resource shape with im_exit set; *count = workgroups that joined the poll.
+ * probe_groups <= 0 picks 2x the HIP occupancy bound.
+ * Replaces get_occupancy_d_ctx (src/host_skeleton/skel.cpp:223-240). */
+int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
+                           int32_t probe_groups, int32_t* count);
+
+/* Barrier round trip at a given occupancy (os_freq,
+ * src/test_suite/os_freq/device/os_freq.cl:8-46): `iterations` x 10
+ * global barriers in one launch. */
+#define WGM_BARRIER_FLAGS 0   /* master/slave flag barrier, one line per WG */
+#define WGM_BARRIER_SENSE 1   /* counter + sense reversal */
+typedef struct wgm_barrier_result {
+  int32_t participating;
+  int32_t barriers;
+  int32_t epoch_errors;  /* WGs that saw a wrong epoch after a barrier */
+  double kernel_ms;      /* HIP event time of the launch */
+  double device_ms;      /* device clock between first and last barrier */
+  double ns_per_barrier;
+} wgm_barrier_result;
+int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t num_wgs,
+                      int32_t iterations, wgm_barrier_result* out);
+
+/* ------------------------------------------------ device-resident graphs */
+typedef struct wgm_graph wgm_graph;
+/* Upload a CSR (and optional uint32 priorities) to HBM. */
+int wgm_graph_create(wgm_runtime* rt, const wgm_csr* g, const uint32_t* prio, wgm_graph** out);
+int wgm_graph_destroy(wgm_graph* g);
+int64_t wgm_graph_bytes(const wgm_graph* g);
+
+/* Launch shape of a standalone persistent kernel. */
+typedef struct wgm_launch {
+  int32_t wg_size;     /* threads per workgroup (64..1024, multiple of 64) */
+  int32_t num_wgs;     /* <= 0: discovered occupancy */
+  int32_t handshake;   /* 1: host flag handshake as in the reference's run_persistent */
+  int32_t reserved;
+} wgm_launch;
+
+typedef struct wgm_run_stats {
+  int32_t participating;   /* discovered workgroups */
+  int32_t iterations;      /* levels / rounds / iterations executed */
+  int64_t edges_scanned;   /* sum of out-degrees of reached vertices (BFS/SSSP) */
+  double kernel_ms;        /* HIP event time of the whole launch */
+  double device_ms;        /* device clock, first to last barrier (excludes discovery) */
+  double host_ms;          /* host handshake flip -> persistent_flag==0 (handshake only) */
+} wgm_run_stats;
+
+/* Standalone persistent apps. Outputs are host arrays of n entries (may be
+ * NULL to leave the result on the device). */
+int wgm_bfs(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc,
+            uint32_t* dist_out, wgm_run_stats* st);
+int wgm_sssp(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc,
+             uint32_t* dist_out, wgm_run_stats* st);
+int wgm_color(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* color_out,
+              wgm_run_stats* st);
+int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
+            wgm_run_stats* st);
+int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations,
+                 const wgm_launch* lc, float* rank_out, wgm_run_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WGM_H */

@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libwgm.so on cuda:0)")
+
+
+@pytest.fixture(scope="session")
+def rt():
+    import gpu_workgroup_managment_amd as wgm
+
+    r = wgm.Runtime(0)
+    yield r
+    r.close()

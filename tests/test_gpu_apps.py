@@ -1,0 +1,115 @@
+"""GPU parity: persistent BFS / SSSP / colour / MIS / PageRank vs the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_workgroup_managment_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _first_diff(a, b):
+    idx = np.flatnonzero(a != b)
+    return f"{len(idx)} differ, first at {idx[0]}: got {a[idx[0]]} want {b[idx[0]]}" if len(idx) else ""
+
+
+@pytest.mark.parametrize("n,pairs,wgs", [(1024, 2048, 4), (4096, 16384, 64), (1 << 16, 1 << 18, 0)])
+def test_bfs_er(rt, n, pairs, wgs):
+    g = G.erdos_renyi(n, pairs, seed=1)
+    dist, st = rt.graph(g).bfs(0, wg_size=256, num_wgs=wgs)
+    ref, levels = oracle.bfs(g.row_ptr, g.col, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["iterations"] == levels
+    reached = ref != oracle.INF
+    assert st["edges_scanned"] == int(g.degrees()[reached].sum())
+    print(st)
+
+
+def test_bfs_grid_closed_form(rt):
+    side = 300
+    g = G.grid(side, side)
+    dist, st = rt.graph(g).bfs(0, wg_size=256, num_wgs=32)
+    r, c = np.divmod(np.arange(side * side), side)
+    assert np.array_equal(dist, (r + c).astype(np.uint32))
+    assert st["iterations"] == 2 * side - 1
+
+
+@pytest.mark.parametrize("wg", [64, 512, 1024])
+def test_bfs_wg_sizes(rt, wg):
+    g = G.rmat(12, 8, seed=7)
+    dist, _ = rt.graph(g).bfs(0, wg_size=wg, num_wgs=16)
+    ref, _ = oracle.bfs(g.row_ptr, g.col, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+
+
+def test_bfs_handshake(rt):
+    g = G.erdos_renyi(4096, 16384, seed=3)
+    dist, st = rt.graph(g).bfs(0, wg_size=256, num_wgs=32, handshake=True)
+    ref, _ = oracle.bfs(g.row_ptr, g.col, 0)
+    assert np.array_equal(dist, ref)
+    assert st["host_ms"] > 0
+
+
+@pytest.mark.parametrize("scale,wgs", [(10, 8), (14, 0)])
+def test_sssp_rmat(rt, scale, wgs):
+    g = G.add_weights(G.rmat(scale, 16, seed=2), 1, 255, 3)
+    dist, st = rt.graph(g).sssp(0, wg_size=256, num_wgs=wgs)
+    ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    print(st)
+
+
+def test_sssp_grid_unit_weights(rt):
+    side = 128
+    g = G.grid(side, side)
+    g.wt = np.ones(g.m, np.int32)
+    dist, _ = rt.graph(g).sssp(0, wg_size=256, num_wgs=16)
+    r, c = np.divmod(np.arange(side * side), side)
+    assert np.array_equal(dist, (r + c).astype(np.uint32))
+
+
+@pytest.mark.parametrize("side,wgs", [(64, 4), (512, 0)])
+def test_color_grid(rt, side, wgs):
+    g = G.add_priorities(G.grid(side, side))
+    col, st = rt.graph(g).color(wg_size=256, num_wgs=wgs)
+    ref, rounds = oracle.color(g.row_ptr, g.col, g.prio)
+    assert np.array_equal(col, ref), _first_diff(col, ref)
+    assert st["iterations"] == rounds
+
+
+def test_color_er_with_degree_one(rt):
+    g = G.add_priorities(G.erdos_renyi(5000, 4000, seed=9))  # many degree-0/1 vertices
+    col, _ = rt.graph(g).color(wg_size=128, num_wgs=8)
+    ref, _ = oracle.color(g.row_ptr, g.col, g.prio)
+    assert np.array_equal(col, ref), _first_diff(col, ref)
+
+
+@pytest.mark.parametrize("side,wgs", [(64, 4), (512, 0)])
+def test_mis_grid(rt, side, wgs):
+    g = G.add_priorities(G.grid(side, side))
+    s, st = rt.graph(g).mis(wg_size=256, num_wgs=wgs)
+    ref, rounds = oracle.mis(g.row_ptr, g.col, g.prio)
+    assert np.array_equal(s, ref), _first_diff(s, ref)
+    assert st["iterations"] == rounds
+
+
+def test_mis_rmat(rt):
+    g = G.rmat(12, 8, seed=5)
+    # make it undirected
+    src = np.repeat(np.arange(g.n), g.degrees())
+    g = G.add_priorities(G.from_edges(g.n, src, g.col, symmetric=True))
+    s, _ = rt.graph(g).mis(wg_size=256, num_wgs=32)
+    ref, _ = oracle.mis(g.row_ptr, g.col, g.prio)
+    assert np.array_equal(s, ref), _first_diff(s, ref)
+
+
+@pytest.mark.parametrize("side,wgs", [(64, 4), (256, 0)])
+def test_pagerank_grid(rt, side, wgs):
+    g = G.grid(side, side)
+    r, st = rt.graph(g).pagerank(0.85, 50, wg_size=256, num_wgs=wgs)
+    ref32 = oracle.pagerank_f32(g.row_ptr, g.col, 0.85, 50)
+    ref64 = oracle.pagerank_f64(g.row_ptr, g.col, 0.85, 50)
+    assert np.array_equal(r.view(np.uint32), ref32.view(np.uint32)), _first_diff(r, ref32)
+    l1 = float(np.abs(r.astype(np.float64) - ref64).sum())
+    print(f"PageRank {side}^2: L1 vs FP64 = {l1:.3e}")
+    assert l1 < 1e-4
