@@ -29,7 +29,7 @@ ERRORS = {
 K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED = range(9)
 GRAPH_RMAT, GRAPH_GRID = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
-BARRIER_FLAGS, BARRIER_SENSE = 0, 1
+BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD = 0, 1, 2
 
 
 class WgmError(RuntimeError):

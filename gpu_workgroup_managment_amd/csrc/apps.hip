@@ -18,98 +18,83 @@ using namespace wgm;
 namespace wgm {
 
 // ====================================================== standalone kernels
-// Shape of each (standalone.cl): discovery; host handshake; barrier; the
-// application loop with global barriers; retire.
+// Shape of each (standalone.cl, e.g. src/lonestar/graph_apps/bfs/device/standalone.cl:247-320):
+// discovery; host handshake; barrier; the application loop with global
+// barriers (here the XCD-aware two-level barrier); retire.
 
 template <bool SSSP>
 __global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDev s, int* iters_out) {
   extern __shared__ int lds[];
-  int n;
-  const int id = discovery_protocol(c, &n);
-  if (id < 0 || c.im_exit) return;
-  standalone_handshake(c, id, n);
-  global_barrier_disc(c, id, n);
-  clock_start(c, id);
+  Team t;
+  if (!standalone_begin(c, t)) return;
   int level = 0;
   while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-    frontier_level<SSSP>(g, s, level, id, n, lds);
-    global_barrier_disc(c, id, n);
+    frontier_level<SSSP>(g, s, level, t.id, t.n, lds);
+    xcd_barrier(c, t);
     level++;
   }
-  clock_end(c, id);
-  if (id == 0 && threadIdx.x == 0) *iters_out = level;
+  clock_end(c, t.id);
+  if (t.id == 0 && threadIdx.x == 0) *iters_out = level;
   standalone_retire(c);
 }
 
 __global__ void __launch_bounds__(1024) k_color(Ctl c, GraphDev g, int32_t* color, int* iters_out) {
-  int n;
-  const int id = discovery_protocol(c, &n);
-  if (id < 0 || c.im_exit) return;
-  standalone_handshake(c, id, n);
-  global_barrier_disc(c, id, n);
-  clock_start(c, id);
+  Team t;
+  if (!standalone_begin(c, t)) return;
   int lo, hi;
-  block_range(g.n, id, n, &lo, &hi);
+  block_range(g.n, t.id, t.n, &lo, &hi);
   int* flags = c.lines;
   int round = 1;
   for (;; round++) {
     const bool any = color_round(g, color, round, lo, hi);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
-    if (id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
-    global_barrier_disc(c, id, n);
+    if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
+    xcd_barrier(c, t);
     if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
   }
-  clock_end(c, id);
-  if (id == 0 && threadIdx.x == 0) *iters_out = round;
+  clock_end(c, t.id);
+  if (t.id == 0 && threadIdx.x == 0) *iters_out = round;
   standalone_retire(c);
 }
 
 __global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, int32_t* s_out, int* iters_out) {
-  int n;
-  const int id = discovery_protocol(c, &n);
-  if (id < 0 || c.im_exit) return;
-  standalone_handshake(c, id, n);
-  global_barrier_disc(c, id, n);
-  clock_start(c, id);
+  Team t;
+  if (!standalone_begin(c, t)) return;
   int lo, hi;
-  block_range(g.n, id, n, &lo, &hi);
+  block_range(g.n, t.id, t.n, &lo, &hi);
   int* flags = c.lines;
   int round = 1;
   for (;; round++) {
     const bool any = mis_phase_a(g, st, s_out, round, lo, hi);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
-    if (id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
-    global_barrier_disc(c, id, n);
+    if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
+    xcd_barrier(c, t);
     if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
     mis_phase_b(g, st, round, lo, hi);
-    global_barrier_disc(c, id, n);
+    xcd_barrier(c, t);
   }
-  clock_end(c, id);
-  if (id == 0 && threadIdx.x == 0) *iters_out = round;
+  clock_end(c, t.id);
+  if (t.id == 0 && threadIdx.x == 0) *iters_out = round;
   standalone_retire(c);
 }
 
 __global__ void __launch_bounds__(1024) k_pagerank(Ctl c, GraphDev g, float* c0, float* c1, float* rank,
                                                    float damping, float base, float r0, int iters,
                                                    int* iters_out) {
-  int n;
-  const int id = discovery_protocol(c, &n);
-  if (id < 0 || c.im_exit) return;
-  standalone_handshake(c, id, n);
-  global_barrier_disc(c, id, n);
-  clock_start(c, id);
+  Team t;
+  if (!standalone_begin(c, t)) return;
   int lo, hi;
-  block_range(g.n, id, n, &lo, &hi);
+  block_range(g.n, t.id, t.n, &lo, &hi);
   pagerank_init(g, c0, r0, lo, hi);
-  global_barrier_disc(c, id, n);
+  xcd_barrier(c, t);
   for (int it = 0; it < iters; it++) {
     const float* cin = (it & 1) ? c1 : c0;
     float* cout = (it & 1) ? c0 : c1;
     pagerank_iter(g, cin, cout, rank, damping, base, it == iters - 1, lo, hi);
-    global_barrier_disc(c, id, n);
+    xcd_barrier(c, t);
   }
-  clock_end(c, id);
-  if (id == 0 && threadIdx.x == 0) *iters_out = iters;
+  clock_end(c, t.id);
+  if (t.id == 0 && threadIdx.x == 0) *iters_out = iters;
   standalone_retire(c);
 }
 

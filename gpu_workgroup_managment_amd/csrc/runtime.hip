@@ -25,7 +25,12 @@ __global__ void k_reset_ctl(Ctl c, int flag_lines, int gid_entries) {
   for (int i = t; i < flag_lines; i += stride) c.flags[i * kLine] = 0;
   for (int i = t; i < gid_entries; i += stride) c.p_group_ids[i] = -1;
   for (int i = t; i < kNumLines; i += stride) c.lines[i * kLine] = 0;
+  for (int i = t; i < kMaxXcd; i += stride) {
+    c.xcd_count[i * kLine] = 0;
+    c.xflags[i * kLine] = 0;
+  }
   if (t == 0) {
+    *c.registered = 0;
     *c.poll = 0;
     *c.count = -1;
     *c.bar_groups = 0;
@@ -44,9 +49,9 @@ __global__ void k_reset_ctl(Ctl c, int flag_lines, int gid_entries) {
 __global__ void __launch_bounds__(1024) k_barrier_bench(Ctl c, int iterations, int variant, int check,
                                                         int* errors, int* arrivals) {
   __shared__ int local_sense;
-  int n;
-  const int id = discovery_protocol(c, &n);
-  if (id < 0 || c.im_exit) return;
+  Team t;
+  if (!discover_team(c, &t) || c.im_exit) return;
+  const int id = t.id, n = t.n;
   if (id == 0 && threadIdx.x == 0) st_relaxed(c.bar_groups, n);
   if (threadIdx.x == 0) local_sense = 0;
   global_barrier_disc(c, id, n);
@@ -56,6 +61,7 @@ __global__ void __launch_bounds__(1024) k_barrier_bench(Ctl c, int iterations, i
     for (int j = 0; j < 10; j++, b++) {
       if (check && threadIdx.x == 0) add_relaxed(arrivals, 1);
       if (variant == 0) global_barrier_disc(c, id, n);
+      else if (variant == 2) xcd_barrier(c, t);
       else global_barrier_sense(c.sense_counter, c.sense_flag, c.bar_groups, &local_sense);
       if (check && threadIdx.x == 0) {
         const int a = ld_relaxed(arrivals);
@@ -274,8 +280,9 @@ int wgm_create(int32_t device, wgm_runtime** out) {
   rt->max_groups = rt->info.num_cus * 32;
   rt->max_grid = rt->max_groups * 2;
   const size_t line_b = kLine * sizeof(int);
-  const size_t n_lines_fixed = 8 + kNumLines + 1;
-  rt->d_ctl_bytes = n_lines_fixed * line_b + (size_t)rt->max_groups * line_b + (size_t)rt->max_grid * sizeof(int);
+  const size_t n_lines_fixed = 8 + kNumLines + 1 + 2 * kMaxXcd + 1;
+  rt->d_ctl_bytes = n_lines_fixed * line_b + (size_t)rt->max_groups * line_b + (size_t)rt->max_grid * sizeof(int) +
+                    (size_t)kMaxXcd * rt->max_groups * sizeof(int);
   e = hipMalloc(&rt->d_ctl, rt->d_ctl_bytes);
   if (e != hipSuccess) return bail(wgm_hip_fail(nullptr, e, "hipMalloc(control arena)"));
   e = hipMemset(rt->d_ctl, 0, rt->d_ctl_bytes);
@@ -293,8 +300,15 @@ int wgm_create(int32_t device, wgm_runtime** out) {
   rt->ctl.lines = (int*)p;
   p += kNumLines * line_b;
   take_line();  // guard
+  rt->ctl.xcd_count = (int*)p;
+  p += kMaxXcd * line_b;
+  rt->ctl.xflags = (int*)p;
+  p += kMaxXcd * line_b;
+  rt->ctl.registered = take_line();
   rt->ctl.flags = (int*)p;
   p += (size_t)rt->max_groups * line_b;
+  rt->ctl.xcd_members = (int*)p;
+  p += (size_t)kMaxXcd * rt->max_groups * sizeof(int);
   rt->ctl.p_group_ids = (int*)p;
   rt->ctl.max_groups = rt->max_groups;
   rt->ctl.window_ticks = (unsigned)((double)rt->window_us * 1e-6 * (double)rt->info.wall_clock_hz);
@@ -388,7 +402,7 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size, int
 int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t num_wgs, int32_t iterations,
                       wgm_barrier_result* out) {
   if (!rt || !out) return fail(rt, WGM_ERR_INVALID, "NULL argument");
-  if (variant != WGM_BARRIER_FLAGS && variant != WGM_BARRIER_SENSE)
+  if (variant != WGM_BARRIER_FLAGS && variant != WGM_BARRIER_SENSE && variant != WGM_BARRIER_XCD)
     return fail(rt, WGM_ERR_INVALID, "unknown barrier variant %d", variant);
   if (iterations < 0) return fail(rt, WGM_ERR_INVALID, "negative iterations");
   int grid = 0;

@@ -84,6 +84,11 @@ struct Ctl {
   int* bar_to_kill;  // query-barrier kill count (bar->to_kill)
   int* sense_counter;
   int* sense_flag;
+  // XCD-aware barrier (two levels: members -> XCD leader, leaders <-> leaders)
+  int* xcd_count;    // kMaxXcd lines: participants per XCD
+  int* registered;   // participants that have recorded their XCD
+  int* xcd_members;  // [kMaxXcd * max_groups]: discovered ids by (XCD, rank in XCD)
+  int* xflags;       // kMaxXcd lines: leader of XCD x arrived at epoch
   // per-launch scratch lines (level counters, stop flags, ...)
   int* lines;        // kNumLines lines
   unsigned long long* clock;  // [0] start tick, [1] end tick
@@ -97,6 +102,7 @@ struct Ctl {
   int handshake;              // standalone host handshake on/off
 };
 constexpr int kNumLines = 16;
+constexpr int kMaxXcd = 16;
 
 // scheduler flag values (src/scheduler_rt/rt_common/cl_scheduler.h:15-21)
 constexpr int DEVICE_SCHEDULER_INIT = 0;
@@ -115,6 +121,25 @@ constexpr int TASK_PERSIST = 2;
 
 constexpr int kPollClosed = 0x40000000;
 constexpr int kPollMask = 0x3fffffff;
+
+// Hardware XCC (XCD) id of the executing workgroup: HW_REG_XCC_ID (id 20), bits [3:0].
+__device__ __forceinline__ int hw_xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kMaxXcd - 1); }
+
+// Wait until this wave's vector memory operations have completed
+// (s_waitcnt vmcnt(0); gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15). A
+// non-tgsplit __syncthreads() does not drain stores, so every wave does this
+// before a barrier whose release is performed by one thread.
+__device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The participating workgroups as seen by one workgroup.
+struct Team {
+  int id;     // discovered id, 0..n-1
+  int n;      // participants
+  int xcc;    // XCD this workgroup runs on
+  int xrank;  // rank among participants on that XCD (0 = XCD leader)
+  int xm;     // participants on that XCD
+  int epoch;  // barriers passed (same on every participant)
+};
 
 // --------------------------------------------------------------- discovery
 // One fetch_add joins the poll; the join order gives dense ids 0..count-1.
@@ -158,6 +183,37 @@ __device__ __forceinline__ int discovery_protocol(const Ctl& c, int* count_out) 
   __syncthreads();
   *count_out = s_disc[1];
   return s_disc[0];
+}
+
+// Discovery plus XCD registration for the two-level barrier: each
+// participant records its XCD and its rank there, then waits until every
+// participant has registered so the per-XCD member counts are final.
+// Returns false for a non-participant.
+__device__ __forceinline__ bool discover_team(const Ctl& c, Team* t) {
+  __shared__ int s_team[3];
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0) return false;
+  if (threadIdx.x == 0) {
+    const int x = hw_xcc_id();
+    const int r = add_relaxed(&c.xcd_count[x * kLine], 1);
+    c.xcd_members[x * c.max_groups + r] = id;
+    fence_release();
+    add_relaxed(c.registered, 1);
+    while (ld_relaxed(c.registered) < n) spin_pause();
+    fence_acquire();
+    s_team[0] = x;
+    s_team[1] = r;
+    s_team[2] = ld_relaxed(&c.xcd_count[x * kLine]);
+  }
+  __syncthreads();
+  t->id = id;
+  t->n = n;
+  t->xcc = s_team[0];
+  t->xrank = s_team[1];
+  t->xm = s_team[2];
+  t->epoch = 0;
+  return true;
 }
 
 // ------------------------------------------------------------ flag barrier
@@ -205,6 +261,7 @@ __device__ __forceinline__ void slave_arrive_wait(int* flags, int id) {
 }
 
 __device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n) {
+  drain_vmem();
   if (n <= 1) {
     __syncthreads();
     return;
@@ -219,6 +276,62 @@ __device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n)
     if (threadIdx.x == 0) slave_arrive_wait(c.flags, id);
     __syncthreads();
   }
+}
+
+// --------------------------------------------------- XCD-aware barrier
+// Barrier over the participants of a fixed team, two levels deep. A member
+// drains its stores into its XCD's L2, publishes arrival 2e-1 in its own line
+// and spins there for 2e. The leader of each XCD (rank 0 there) collects its
+// members, writes back its XCD's L2 once (agent release) -- the L2 is shared
+// by the XCD, so this publishes every member's stores -- meets the other XCD
+// leaders through one line each, and releases its members; every workgroup
+// then performs its own agent acquire. Per barrier that is one L2 writeback
+// per XCD instead of one per workgroup, and no workgroup polls more than one
+// XCD's worth of lines. Same contract as global_barrier_disc
+// (src/scheduler_rt/rt_device/iw_barrier.cl:63-109).
+// Measured on MI355X: the XCD L2s are not coherent with each other (without
+// the leaders' writeback, readers on other XCDs see stale data) and an
+// L1-only invalidate (buffer_inv sc0) does not act outside threadgroup-split
+// mode, so members keep the full agent-scope acquire.
+__device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
+  drain_vmem();
+  if (t.n <= 1) {
+    __syncthreads();
+    return;
+  }
+  t.epoch++;
+  const int arrive = 2 * t.epoch - 1, release = 2 * t.epoch;
+  const int* members = c.xcd_members + t.xcc * c.max_groups;
+  __syncthreads();
+  if (t.xrank != 0) {
+    if (threadIdx.x == 0) {
+      int* f = &c.flags[t.id * kLine];
+      st_relaxed(f, arrive);
+      while (ld_relaxed(f) != release) {
+      }
+      fence_acquire();
+    }
+  } else {
+    for (int r = 1 + (int)threadIdx.x; r < t.xm; r += blockDim.x) {
+      const int* f = &c.flags[members[r] * kLine];
+      while (ld_relaxed(f) != arrive) {
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      fence_release();
+      st_relaxed(&c.xflags[t.xcc * kLine], t.epoch);
+    }
+    if (threadIdx.x < kMaxXcd && ld_relaxed(&c.xcd_count[threadIdx.x * kLine]) > 0) {
+      while (ld_relaxed(&c.xflags[threadIdx.x * kLine]) < t.epoch) {
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) fence_acquire();
+    __syncthreads();
+    for (int r = 1 + (int)threadIdx.x; r < t.xm; r += blockDim.x) st_relaxed(&c.flags[members[r] * kLine], release);
+  }
+  __syncthreads();
 }
 
 // -------------------------------------------------- sense-reversal barrier
@@ -273,6 +386,17 @@ __device__ __forceinline__ void clock_start(const Ctl& c, int id) {
 }
 __device__ __forceinline__ void clock_end(const Ctl& c, int id) {
   if (id == 0 && threadIdx.x == 0) c.clock[1] = wall_clock64();
+}
+
+// Prologue of a standalone persistent kernel: discovery (+XCD registration),
+// optional host handshake, a first barrier, start of the device clock.
+// Returns false for a workgroup that does not take part.
+__device__ __forceinline__ bool standalone_begin(const Ctl& c, Team& t) {
+  if (!discover_team(c, &t) || c.im_exit) return false;
+  standalone_handshake(c, t.id, t.n);
+  xcd_barrier(c, t);
+  clock_start(c, t.id);
+  return true;
 }
 
 // --------------------------------------------------------- workgroup scans

@@ -127,8 +127,9 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
 /* Barrier round trip at a given occupancy (os_freq,
  * src/test_suite/os_freq/device/os_freq.cl:8-46): `iterations` x 10
  * global barriers in one launch. */
-#define WGM_BARRIER_FLAGS 0   /* master/slave flag barrier, one line per WG */
+#define WGM_BARRIER_FLAGS 0   /* flat master/slave flag barrier, one line per WG */
 #define WGM_BARRIER_SENSE 1   /* counter + sense reversal */
+#define WGM_BARRIER_XCD 2     /* two-level: members -> XCD leader, leaders <-> leaders */
 typedef struct wgm_barrier_result {
   int32_t participating;
   int32_t barriers;
