@@ -109,6 +109,36 @@ class RunStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("wg_size", ctypes.c_int32),
+        ("wgs_per_cu", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("team_edges", ctypes.c_int32),
+        ("max_team", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("gteps", ctypes.c_double),
+        ("edges_scanned", ctypes.c_int64),
+        ("participating", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -143,6 +173,13 @@ SIGNATURES = {
     "wgm_color": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_mis": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_pagerank": (_i32, [_vp, _vp, ctypes.c_float, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_batch_create": (_i32, [_vp, _i32, _P(Csr), _vp, _vp, _P(_vp)]),
+    "wgm_batch_destroy": (_i32, [_vp]),
+    "wgm_batch_bytes": (_i64, [_vp]),
+    "wgm_batch_run": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
+    "wgm_batch_run_relaunch": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
+    "wgm_batch_result": (_i32, [_vp, _i32, _vp, _P(_i32), _P(_i64)]),
+    "wgm_batch_autotune": (_i32, [_vp, _vp, _i32, _P(BatchConfig), _P(BatchStats)]),
 }
 
 _lib = None

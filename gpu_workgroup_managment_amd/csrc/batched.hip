@@ -1,0 +1,654 @@
+// batched.hip -- many graphs in one persistent launch (device work queue),
+// the per-graph relaunch baseline, hipGraph capture and the autotuner.
+//
+// Not in the reference, which runs one graph per process
+// (src/host_skeleton/skel.cpp:250-385). The per-graph body is the same
+// frontier relaxation as the standalone BFS / SSSP kernels
+// (src/lonestar/graph_apps/bfs/device/bfs.cl:214-302,
+// src/lonestar/graph_apps/sssp/device/sssp.cl:262-318).
+//
+// Work distribution: job j (one graph + query) owns `team_j` consecutive
+// tickets; a workgroup that draws ticket t serves job j = the last job whose
+// first ticket is <= t, as team rank t - first_j. Tickets are drawn in job
+// order (largest graphs first), so at most one team is partially formed at
+// any time and its missing members are the next free workgroups: with every
+// puller co-resident (discovery), no team waits forever.
+
+#include "apps_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using namespace wgm;
+
+namespace wgm {
+
+struct BatchJob {
+  long long row_off;  // into the row_ptr pool
+  long long col_off;  // into the col / wt pools
+  long long v_off;    // into the per-vertex pools (dist, stamp)
+  long long q_off;    // into the queue pool (2 * (n + 1) entries)
+  int n, m, query, team, src, ticket;
+  int index;          // caller's graph index
+  int pad;
+};
+
+struct BatchDev {
+  const BatchJob* jobs;
+  int njobs;
+  int total_tickets;
+  const int32_t* row;
+  const int32_t* col;
+  const int32_t* wt;
+  uint32_t* dist;
+  uint32_t* stamp;
+  int32_t* q;
+  int* counts;     // 3 lines per job
+  int* claims;     // 3 lines per job
+  int chunk;
+  int* tbar;       // 2 lines per job: arrival counter, generation
+  int* ticket;     // one line
+  int* iters;      // per job
+  int max_team;    // largest team in the job table
+};
+
+// Counter barrier of a team of k workgroups (k small). Arrival is an acq_rel
+// fetch_add on a monotonic counter; the arrival that completes epoch e
+// publishes generation e; everyone then acquires.
+__device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epoch) {
+  drain_vmem();
+  __syncthreads();
+  epoch++;
+  if (threadIdx.x == 0) {
+    const int old = add_acq_rel(cnt, 1);
+    if (old == epoch * k - 1) {
+      st_release(gen, epoch);
+    } else {
+      while (ld_relaxed(gen) < epoch) {
+      }
+      fence_acquire();
+    }
+  }
+  __syncthreads();
+}
+
+// A graph served by one workgroup: level counters live in LDS and the
+// workgroup's own barrier orders the levels; all atomics are workgroup scope.
+template <bool SSSP>
+__device__ void run_single(const GraphDev& g, const FrontierDev& s, int src, int* lds, int* s_cnt, int* iters_out) {
+  for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
+    s.dist[v] = (v == src) ? 0u : kInf;
+    if (SSSP) s.stamp[v] = 0u;
+  }
+  if (threadIdx.x == 0) {
+    s.q0[0] = src;
+    s_cnt[0] = 1;
+    s_cnt[1] = 0;
+    s_cnt[2] = 0;
+  }
+  __syncthreads();
+  const int bd = blockDim.x;
+  int level = 0;
+  for (;; level++) {
+    const int in_n = s_cnt[level % 3];
+    if (in_n == 0) break;
+    const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
+    int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+    for (int a = 0; a < in_n; a += bd)
+      frontier_batch<SSSP, true>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3], (uint32_t)level + 1u,
+                                 lds);
+    if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *iters_out = level;
+}
+
+// A graph served by a team of k > 1 workgroups.
+template <bool SSSP>
+__device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int rank, int k, int* tcnt, int* tgen,
+                         int* lds, int* iters_out) {
+  int epoch = 0;
+  int lo, hi;
+  block_range(g.n, rank, k, &lo, &hi);
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    s.dist[v] = (v == src) ? 0u : kInf;
+    if (SSSP) s.stamp[v] = 0u;
+  }
+  if (rank == 0 && threadIdx.x == 0) {
+    s.q0[0] = src;
+    st_relaxed(&s.counts[0], 1);
+    st_relaxed(&s.counts[kLine], 0);
+    st_relaxed(&s.counts[2 * kLine], 0);
+    for (int l = 0; l < 3; l++) st_relaxed(&s.claims[l * kLine], 0);
+  }
+  team_barrier(tcnt, tgen, k, epoch);
+  int level = 0;
+  while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
+    frontier_level<SSSP>(g, s, level, rank, k, lds);
+    team_barrier(tcnt, tgen, k, epoch);
+    level++;
+  }
+  if (rank == 0 && threadIdx.x == 0) *iters_out = level;
+}
+
+__global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
+  extern __shared__ int lds[];
+  __shared__ int s_job[2];
+  __shared__ int s_cnt[3];
+  int n;
+  const int id = discovery_protocol(c, &n);
+  if (id < 0 || c.im_exit) return;
+  // A team larger than the participants could never form: report and leave.
+  if (n < b.max_team) {
+    if (id == 0 && threadIdx.x == 0) st_relaxed(&c.lines[kLine], n);
+    return;
+  }
+  clock_start(c, id);
+  while (true) {
+    if (threadIdx.x == 0) {
+      const int t = add_relaxed(b.ticket, 1);
+      int j = -1, r = 0;
+      if (t < b.total_tickets) {
+        int lo = 0, hi = b.njobs - 1;
+        while (lo < hi) {  // last job with ticket <= t
+          const int mid = (lo + hi + 1) >> 1;
+          if (b.jobs[mid].ticket <= t) lo = mid; else hi = mid - 1;
+        }
+        j = lo;
+        r = t - b.jobs[j].ticket;
+      }
+      s_job[0] = j;
+      s_job[1] = r;
+    }
+    __syncthreads();
+    const int j = s_job[0], r = s_job[1];
+    __syncthreads();
+    if (j < 0) break;
+    const BatchJob jb = b.jobs[j];
+    GraphDev g{};
+    g.row_ptr = b.row + jb.row_off;
+    g.col = b.col + jb.col_off;
+    g.wt = b.wt ? b.wt + jb.col_off : nullptr;
+    g.n = jb.n;
+    g.m = jb.m;
+    FrontierDev s{};
+    s.dist = b.dist + jb.v_off;
+    s.stamp = b.stamp + jb.v_off;
+    s.q0 = b.q + jb.q_off;
+    s.q1 = b.q + jb.q_off + jb.n + 1;
+    s.counts = b.counts + (size_t)j * 3 * kLine;
+    s.claims = b.claims + (size_t)j * 3 * kLine;
+    s.chunk = b.chunk;
+    const bool sssp = jb.query == WGM_QUERY_SSSP;
+    if (jb.team <= 1) {
+      if (sssp) run_single<true>(g, s, jb.src, lds, s_cnt, &b.iters[j]);
+      else run_single<false>(g, s, jb.src, lds, s_cnt, &b.iters[j]);
+    } else {
+      int* tcnt = b.tbar + (size_t)j * 2 * kLine;
+      int* tgen = tcnt + kLine;
+      if (sssp) run_team<true>(g, s, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
+      else run_team<false>(g, s, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
+    }
+    __syncthreads();
+  }
+  // the last workgroup to leave stamps the end of the batch
+  if (threadIdx.x == 0) {
+    fence_release();
+    if (add_acq_rel(&c.lines[0], 1) == n - 1) c.clock[1] = wall_clock64();
+  }
+}
+
+// Per-launch reset of the work queue and team barriers (capture-safe: a
+// kernel, not a memset).
+__global__ void k_batch_reset(BatchDev b) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int i = t; i < b.njobs * 2; i += gridDim.x * blockDim.x) b.tbar[i * kLine] = 0;
+  if (t == 0) *b.ticket = 0;
+}
+
+// Sum of out-degrees of reached vertices per job: one workgroup per job.
+__global__ void k_batch_reached(BatchDev b, unsigned long long* per_job) {
+  for (int j = blockIdx.x; j < b.njobs; j += gridDim.x) {
+    const BatchJob jb = b.jobs[j];
+    unsigned long long local = 0;
+    for (int v = threadIdx.x; v < jb.n; v += blockDim.x)
+      if (b.dist[jb.v_off + v] != kInf) local += (unsigned long long)(b.row[jb.row_off + v + 1] - b.row[jb.row_off + v]);
+    for (int d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(&per_job[j], local);
+  }
+}
+
+}  // namespace wgm
+
+#define fail wgm_fail
+#define HIPCHK WGM_HIPCHK
+
+struct wgm_batch {
+  wgm_runtime* rt = nullptr;
+  int count = 0;
+  std::vector<int32_t> n, query, src;
+  std::vector<int64_t> m, row_off, col_off, v_off, q_off;
+  int64_t total_rows = 0, total_edges = 0, total_v = 0, total_q = 0;
+  bool weighted = false;
+  int32_t *row = nullptr, *col = nullptr, *wt = nullptr, *q = nullptr, *counts = nullptr, *tbar = nullptr;
+  int32_t* claims = nullptr;
+  BatchJob* d_jobs_one = nullptr;  // relaunch baseline: the same jobs, tickets from 0
+  int chunk = 0;
+  int32_t *ticket = nullptr, *iters = nullptr;
+  uint32_t *dist = nullptr, *stamp = nullptr;
+  BatchJob* d_jobs = nullptr;
+  unsigned long long* d_reached = nullptr;
+  std::vector<BatchJob> jobs;           // current job table (LPT order)
+  wgm_batch_config jobs_cfg{};          // config and grid the device job table was built for
+  int jobs_grid = -1;
+  std::vector<int64_t> reached;         // per caller index, after a run
+  // captured launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  wgm_batch_config graph_cfg{};
+  int graph_grid = 0;
+};
+
+namespace {
+
+int team_size(const wgm_batch_config& cfg, int64_t m) {
+  const int64_t te = cfg.team_edges > 0 ? cfg.team_edges : (1 << 18);
+  int64_t k = (m + te - 1) / te;
+  if (k < 1) k = 1;
+  if (k > cfg.max_team) k = cfg.max_team;
+  return (int)k;
+}
+
+BatchDev dev_view(wgm_batch* b) {
+  BatchDev d{};
+  d.jobs = b->d_jobs;
+  d.njobs = b->count;
+  d.total_tickets = 0;
+  d.max_team = 1;
+  for (const auto& j : b->jobs) {
+    d.total_tickets += j.team;
+    d.max_team = std::max(d.max_team, j.team);
+  }
+  d.row = b->row;
+  d.col = b->col;
+  d.wt = b->weighted ? b->wt : nullptr;
+  d.dist = b->dist;
+  d.stamp = b->stamp;
+  d.q = b->q;
+  d.counts = b->counts;
+  d.claims = b->claims;
+  d.chunk = b->chunk;
+  d.tbar = b->tbar;
+  d.ticket = b->ticket;
+  d.iters = b->iters;
+  return d;
+}
+
+int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out) {
+  wgm_batch_config c{};
+  if (in) c = *in;
+  if (c.wg_size <= 0) c.wg_size = 256;
+  if (c.wgs_per_cu <= 0) c.wgs_per_cu = 4;
+  if (c.max_team <= 0) c.max_team = 16;
+  if (c.team_edges <= 0) c.team_edges = 1 << 18;
+  if (c.chunk <= 0) c.chunk = c.wg_size;
+  int rc = wgm_check_wg(rt, c.wg_size);
+  if (rc) return rc;
+  if (c.wgs_per_cu > 32 || c.max_team > 1024) return fail(rt, WGM_ERR_INVALID, "bad batch config");
+  *out = c;
+  return WGM_OK;
+}
+
+// Job table for a config: LPT order (edges descending), team sizes, tickets.
+int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
+  std::vector<int> order(b->count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->m[a] > b->m[c]; });
+  b->jobs.assign(b->count, BatchJob{});
+  int ticket = 0;
+  for (int k = 0; k < b->count; k++) {
+    const int i = order[k];
+    BatchJob& j = b->jobs[k];
+    j.row_off = b->row_off[i];
+    j.col_off = b->col_off[i];
+    j.v_off = b->v_off[i];
+    j.q_off = b->q_off[i];
+    j.n = b->n[i];
+    j.m = (int)b->m[i];
+    j.query = b->query[i];
+    j.team = std::min(team_size(cfg, b->m[i]), std::max(1, grid));
+    j.src = b->src[i];
+    j.ticket = ticket;
+    j.index = i;
+    ticket += j.team;
+  }
+  b->chunk = cfg.chunk;
+  b->jobs_cfg = cfg;
+  b->jobs_grid = grid;
+  WGM_HIPCHK(b->rt, hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * b->count, hipMemcpyHostToDevice));
+  std::vector<BatchJob> one = b->jobs;
+  for (auto& j : one) j.ticket = 0;
+  WGM_HIPCHK(b->rt, hipMemcpy(b->d_jobs_one, one.data(), sizeof(BatchJob) * b->count, hipMemcpyHostToDevice));
+  return WGM_OK;
+}
+
+int batch_grid(wgm_runtime* rt, const wgm_batch_config& cfg, int* grid) {
+  int disc = 0;
+  int rc = wgm_resolve_grid(rt, WGM_K_BATCHED, cfg.wg_size, 0, &disc);
+  if (rc) return rc;
+  int g = cfg.wgs_per_cu * rt->info.num_cus;
+  *grid = std::min(g, disc);
+  return WGM_OK;
+}
+
+// Enqueue one persistent batched launch (reset kernels + the kernel).
+int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStream_t st) {
+  wgm_runtime* rt = b->rt;
+  BatchDev d = dev_view(b);
+  int rc = wgm_enqueue_reset_ctl(rt, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_batched, dim3(grid), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size), st, rt->ctl, d);
+  WGM_HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
+int collect_results(wgm_batch* b, wgm_batch_stats* st) {
+  wgm_runtime* rt = b->rt;
+  BatchDev d = dev_view(b);
+  WGM_HIPCHK(rt, hipMemsetAsync(b->d_reached, 0, sizeof(unsigned long long) * b->count, rt->stream));
+  hipLaunchKernelGGL(k_batch_reached, dim3(std::min(b->count, 1024)), dim3(256), 0, rt->stream, d, b->d_reached);
+  WGM_HIPCHK(rt, hipGetLastError());
+  std::vector<unsigned long long> per_job(b->count);
+  WGM_HIPCHK(rt, hipMemcpyAsync(per_job.data(), b->d_reached, sizeof(unsigned long long) * b->count,
+                                hipMemcpyDeviceToHost, rt->stream));
+  WGM_HIPCHK(rt, hipStreamSynchronize(rt->stream));
+  b->reached.assign(b->count, 0);
+  int64_t total = 0;
+  for (int k = 0; k < b->count; k++) {
+    b->reached[b->jobs[k].index] = (int64_t)per_job[k];
+    total += (int64_t)per_job[k];
+  }
+  if (st) st->edges_scanned = total;
+  return WGM_OK;
+}
+
+}  // namespace
+
+// Extension hooks used by runtime.hip
+bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
+  if (kernel == WGM_K_BATCHED) {
+    *out = KernelInfo{(const void*)&k_batched, 1};
+    return true;
+  }
+  return wgm_sched_kernel_info(kernel, out);
+}
+
+int wgm_discover_extra(wgm_runtime* rt, int kernel, int grid, int wg_size) {
+  if (kernel == WGM_K_BATCHED) {
+    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(wg_size), frontier_lds_bytes(wg_size), rt->stream, rt->ctl,
+                       BatchDev{});
+    return WGM_OK;
+  }
+  return wgm_sched_probe(rt, kernel, grid, wg_size);
+}
+
+extern "C" {
+
+int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const int32_t* queries,
+                     const int32_t* sources, wgm_batch** out) {
+  if (!rt || !graphs || !queries || !out || count < 1) return fail(rt, WGM_ERR_INVALID, "bad batch arguments");
+  wgm_batch* b = new wgm_batch();
+  b->rt = rt;
+  b->count = count;
+  auto bail = [&](int rc) { wgm_batch_destroy(b); return rc; };
+  for (int i = 0; i < count; i++) {
+    const wgm_csr& g = graphs[i];
+    if (g.n < 1 || g.m < 0 || g.m > INT32_MAX || !g.row_ptr || (g.m && !g.col) || g.row_ptr[0] != 0 ||
+        g.row_ptr[g.n] != g.m)
+      return bail(fail(rt, WGM_ERR_INVALID, "graph %d: bad CSR", i));
+    if (queries[i] != WGM_QUERY_BFS && queries[i] != WGM_QUERY_SSSP)
+      return bail(fail(rt, WGM_ERR_INVALID, "graph %d: bad query", i));
+    if (queries[i] == WGM_QUERY_SSSP && !g.wt) return bail(fail(rt, WGM_ERR_INVALID, "graph %d: SSSP needs weights", i));
+    const int s = sources ? sources[i] : 0;
+    if (s < 0 || s >= g.n) return bail(fail(rt, WGM_ERR_INVALID, "graph %d: bad source", i));
+    if (g.wt) b->weighted = true;
+    b->n.push_back(g.n);
+    b->m.push_back(g.m);
+    b->query.push_back(queries[i]);
+    b->src.push_back(s);
+    b->row_off.push_back(b->total_rows);
+    b->col_off.push_back(b->total_edges);
+    b->v_off.push_back(b->total_v);
+    b->q_off.push_back(b->total_q);
+    b->total_rows += g.n + 1;
+    b->total_edges += g.m;
+    b->total_v += g.n;
+    b->total_q += 2 * ((int64_t)g.n + 1);
+  }
+  auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 4); };
+  hipError_t e = alloc((void**)&b->row, sizeof(int32_t) * b->total_rows);
+  if (e == hipSuccess) e = alloc((void**)&b->col, sizeof(int32_t) * b->total_edges);
+  if (e == hipSuccess && b->weighted) e = alloc((void**)&b->wt, sizeof(int32_t) * b->total_edges);
+  if (e == hipSuccess) e = alloc((void**)&b->dist, sizeof(uint32_t) * b->total_v);
+  if (e == hipSuccess) e = alloc((void**)&b->stamp, sizeof(uint32_t) * b->total_v);
+  if (e == hipSuccess) e = alloc((void**)&b->q, sizeof(int32_t) * b->total_q);
+  if (e == hipSuccess) e = alloc((void**)&b->counts, sizeof(int32_t) * 3 * kLine * count);
+  if (e == hipSuccess) e = alloc((void**)&b->claims, sizeof(int32_t) * 3 * kLine * count);
+  if (e == hipSuccess) e = alloc((void**)&b->d_jobs_one, sizeof(BatchJob) * count);
+  if (e == hipSuccess) e = alloc((void**)&b->tbar, sizeof(int32_t) * 2 * kLine * count);
+  if (e == hipSuccess) e = alloc((void**)&b->ticket, sizeof(int32_t) * kLine);
+  if (e == hipSuccess) e = alloc((void**)&b->iters, sizeof(int32_t) * count);
+  if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
+  if (e == hipSuccess) e = alloc((void**)&b->d_reached, sizeof(unsigned long long) * count);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(batch pools)"));
+  // one staging copy per pool
+  std::vector<int32_t> stage;
+  stage.reserve((size_t)std::max(b->total_rows, b->total_edges));
+  for (int i = 0; i < count; i++) stage.insert(stage.end(), graphs[i].row_ptr, graphs[i].row_ptr + graphs[i].n + 1);
+  e = hipMemcpy(b->row, stage.data(), sizeof(int32_t) * stage.size(), hipMemcpyHostToDevice);
+  stage.clear();
+  for (int i = 0; i < count && e == hipSuccess; i++) stage.insert(stage.end(), graphs[i].col, graphs[i].col + graphs[i].m);
+  if (e == hipSuccess && !stage.empty()) e = hipMemcpy(b->col, stage.data(), sizeof(int32_t) * stage.size(), hipMemcpyHostToDevice);
+  if (b->weighted) {
+    stage.clear();
+    for (int i = 0; i < count; i++) {
+      if (graphs[i].wt) stage.insert(stage.end(), graphs[i].wt, graphs[i].wt + graphs[i].m);
+      else stage.insert(stage.end(), (size_t)graphs[i].m, 1);
+    }
+    if (e == hipSuccess && !stage.empty()) e = hipMemcpy(b->wt, stage.data(), sizeof(int32_t) * stage.size(), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "batch upload"));
+  *out = b;
+  return WGM_OK;
+}
+
+int wgm_batch_destroy(wgm_batch* b) {
+  if (!b) return WGM_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->stamp,
+                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete b;
+  return WGM_OK;
+}
+
+int64_t wgm_batch_bytes(const wgm_batch* b) {
+  if (!b) return 0;
+  return 4 * (b->total_rows + b->total_edges * (b->weighted ? 2 : 1));
+}
+
+int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in, wgm_batch_stats* st) {
+  if (!rt || !b || b->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/batch");
+  wgm_batch_config cfg;
+  int rc = normalize(rt, cfg_in, &cfg);
+  if (rc) return rc;
+  int grid = 0;
+  rc = batch_grid(rt, cfg, &grid);
+  if (rc) return rc;
+  const bool same = b->exec && std::memcmp(&cfg, &b->graph_cfg, sizeof(cfg)) == 0 && grid == b->graph_grid;
+  // The captured launch reads the device job table by pointer: make sure it
+  // holds this config's table (another run may have rebuilt it meanwhile).
+  const bool table_ok = std::memcmp(&cfg, &b->jobs_cfg, sizeof(cfg)) == 0 && grid == b->jobs_grid;
+  if (!same || !cfg.use_graph || !table_ok) {
+    rc = build_jobs(b, cfg, grid);
+    if (rc) return rc;
+  }
+  rt->ctl.im_exit = 0;
+  rt->ctl.handshake = 0;
+  if (cfg.use_graph && !same) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    HIPCHK(rt, hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal));
+    int rc2 = enqueue_batched(b, cfg, grid, rt->stream);
+    hipError_t e = hipStreamEndCapture(rt->stream, &b->graph);
+    if (rc2) return rc2;
+    if (e != hipSuccess) return wgm_hip_fail(rt, e, "hipStreamEndCapture");
+    HIPCHK(rt, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_cfg = cfg;
+    b->graph_grid = grid;
+  }
+  HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+  if (cfg.use_graph) {
+    HIPCHK(rt, hipGraphLaunch(b->exec, rt->stream));
+  } else {
+    rc = enqueue_batched(b, cfg, grid, rt->stream);
+    if (rc) return rc;
+  }
+  wgm_run_stats rs{};
+  rc = wgm_finish_launch(rt, &rs);
+  if (rc) return rc;
+  int short_team = 0;
+  HIPCHK(rt, hipMemcpy(&short_team, rt->ctl.lines + kLine, sizeof(int), hipMemcpyDeviceToHost));
+  if (short_team)
+    return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "only %d workgroups discovered for teams of up to %d", short_team,
+                dev_view(b).max_team);
+  wgm_batch_stats local{};
+  local.wall_ms = rs.kernel_ms;
+  local.device_ms = rs.device_ms;
+  local.participating = rs.participating;
+  local.launches = 1;
+  local.grid = grid;
+  rc = collect_results(b, &local);
+  if (rc) return rc;
+  local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+// Baseline: the same graphs, one persistent launch per graph, back to back
+// on the stream (each launch runs discovery and serves exactly one job with
+// a team of the same size the batched launch would use).
+int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in, wgm_batch_stats* st) {
+  if (!rt || !b || b->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/batch");
+  wgm_batch_config cfg;
+  int rc = normalize(rt, cfg_in, &cfg);
+  if (rc) return rc;
+  int grid = 0;
+  rc = batch_grid(rt, cfg, &grid);
+  if (rc) return rc;
+  rc = build_jobs(b, cfg, grid);
+  if (rc) return rc;
+  rt->ctl.im_exit = 0;
+  rt->ctl.handshake = 0;
+  BatchDev d = dev_view(b);
+  HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+  for (int k = 0; k < b->count; k++) {
+    BatchDev one = d;
+    one.jobs = b->d_jobs_one + k;  // same job, tickets numbered from 0
+    one.njobs = 1;
+    one.total_tickets = b->jobs[k].team;
+    one.max_team = b->jobs[k].team;
+    one.counts = d.counts + (size_t)k * 3 * kLine;
+    one.claims = d.claims + (size_t)k * 3 * kLine;
+    one.tbar = d.tbar + (size_t)k * 2 * kLine;
+    one.iters = d.iters + k;
+    rc = wgm_enqueue_reset_ctl(rt, rt->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, rt->stream, one);
+    hipLaunchKernelGGL(k_batched, dim3(b->jobs[k].team), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size),
+                       rt->stream, rt->ctl, one);
+  }
+  HIPCHK(rt, hipGetLastError());
+  wgm_run_stats rs{};
+  rc = wgm_finish_launch(rt, &rs);
+  if (rc) return rc;
+  wgm_batch_stats local{};
+  local.wall_ms = rs.kernel_ms;
+  local.launches = b->count;
+  local.grid = grid;
+  rc = collect_results(b, &local);
+  if (rc) return rc;
+  local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* iterations, int64_t* edges_scanned) {
+  if (!b || index < 0 || index >= b->count) return fail(b ? b->rt : nullptr, WGM_ERR_INVALID, "bad batch index");
+  wgm_runtime* rt = b->rt;
+  if (dist_out)
+    HIPCHK(rt, hipMemcpy(dist_out, b->dist + b->v_off[index], sizeof(uint32_t) * b->n[index], hipMemcpyDeviceToHost));
+  if (iterations) {
+    int pos = -1;
+    for (int k = 0; k < b->count; k++)
+      if (b->jobs.size() == (size_t)b->count && b->jobs[k].index == index) pos = k;
+    int it = -1;
+    if (pos >= 0) HIPCHK(rt, hipMemcpy(&it, b->iters + pos, sizeof(int), hipMemcpyDeviceToHost));
+    *iterations = it;
+  }
+  if (edges_scanned) *edges_scanned = b->reached.empty() ? -1 : b->reached[index];
+  return WGM_OK;
+}
+
+// Grid search over workgroup size, workgroups per CU and team granularity;
+// keeps the fastest configuration (each candidate timed `reps` times, best kept).
+int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best_out,
+                       wgm_batch_stats* best_stats) {
+  if (!rt || !b || !best_out) return fail(rt, WGM_ERR_INVALID, "bad autotune arguments");
+  if (reps < 1) reps = 1;
+  const int wgs[] = {64, 128, 256, 512, 1024};
+  const int per_cu[] = {1, 2, 4, 8};
+  const int team_edges[] = {1 << 16, 1 << 18, 1 << 20};
+  const int chunks[] = {32, 256, 4096};
+  wgm_batch_config best{};
+  wgm_batch_stats bst{};
+  double best_ms = 1e30;
+  for (int wg : wgs)
+    for (int pc : per_cu) {
+      if (wg * pc > 2048) continue;  // CU thread capacity
+      for (int te : team_edges)
+        for (int ch : chunks) {
+          if (ch != 256 && te != (1 << 18)) continue;  // chunk only swept at the middle team size
+          wgm_batch_config c{};
+          c.wg_size = wg;
+          c.wgs_per_cu = pc;
+          c.team_edges = te;
+          c.chunk = ch;
+          c.max_team = 16;
+          c.use_graph = 0;
+          for (int r = 0; r < reps; r++) {
+            wgm_batch_stats s{};
+            int rc = wgm_batch_run(rt, b, &c, &s);
+            if (rc) return rc;
+            if (s.wall_ms < best_ms) {
+              best_ms = s.wall_ms;
+              best = c;
+              bst = s;
+            }
+          }
+        }
+    }
+  *best_out = best;
+  if (best_stats) *best_stats = bst;
+  return WGM_OK;
+}
+
+}  // extern "C"

@@ -30,88 +30,123 @@ struct GraphDev {
 // Frontier state. Level L's frontier is q[L & 1] with its length in
 // counts[(L % 3) * kLine]; three counters let one barrier end a level: during
 // level L the workgroups read counter L%3, push into (L+1)%3 and rank 0 zeroes
-// (L+2)%3, whose last readers finished before the previous barrier.
+// (L+2)%3, whose last readers finished before the previous barrier. The
+// claim counters (dynamic chunking) rotate the same way.
 struct FrontierDev {
   uint32_t* dist;
   uint32_t* stamp;  // SSSP: last level v was queued for (dedups pushes)
   int32_t* q0;
   int32_t* q1;
   int* counts;      // 3 lines
+  int* claims;      // 3 lines (dynamic chunk claiming), may be null
+  int chunk;        // frontier vertices per claim; <= 0: static round-robin chunks
 };
 
-// Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17]
-__host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 17) * 4; }
+// Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17], claim[1]
+__host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 18) * 4; }
 
-// One level of frontier relaxation over q[level & 1]. Chunks of bd frontier
-// vertices go to team workgroups round-robin; inside a chunk the workgroup
-// scans the degrees and spreads the chunk's edges evenly over its threads,
-// each finding its source vertex by binary search in LDS, so a high-degree
-// vertex does not serialise one thread. BFS claims a vertex with one CAS
-// (INF -> level+1) and pushes it exactly once; SSSP lowers dist with
-// atomic min and pushes a vertex at most once per level through `stamp`.
-// Reference: processnode2 / processedge2 / wl_push_1item
-// (src/lonestar/graph_apps/bfs/device/bfs.cl:82-212).
-template <bool SSSP>
-__device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
-                                               int rank, int nteam, int* lds) {
+// Relax the edges of frontier entries in_q[a, b) (b - a <= blockDim.x).
+// The workgroup scans the entries' degrees and spreads their edges evenly
+// over its threads, each finding its source entry by binary search in LDS,
+// so a high-degree vertex does not serialise one thread. BFS claims a vertex
+// with one CAS (INF -> level+1) and pushes it exactly once; SSSP lowers dist
+// with atomic min and pushes a vertex at most once per level through
+// `stamp`. Pushes are wave-aggregated: one counter atomic per wave.
+// LOCAL: the push counter is in LDS and only this workgroup touches the
+// graph (workgroup-scope atomics). Reference: processnode2 / processedge2 /
+// wl_push_1item (src/lonestar/graph_apps/bfs/device/bfs.cl:82-212).
+template <bool SSSP, bool LOCAL>
+__device__ __forceinline__ void frontier_batch(const GraphDev& g, const FrontierDev& s, const int32_t* in_q, int a,
+                                               int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds) {
+  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
   const int bd = blockDim.x, tid = threadIdx.x;
   int* s_end = lds;
   int* s_beg = lds + bd;
   uint32_t* s_du = (uint32_t*)(lds + 2 * bd);
   int* s_wave = lds + 3 * bd;
+  const int i = a + tid;
+  int beg = 0, deg = 0;
+  uint32_t du = 0;
+  if (i < b) {
+    const int v = in_q[i];
+    beg = g.row_ptr[v];
+    deg = g.row_ptr[v + 1] - beg;
+    if (SSSP) du = s.dist[v];
+  }
+  int total;
+  const int excl = wg_exclusive_scan(deg, s_wave, &total);
+  s_end[tid] = excl + deg;
+  s_beg[tid] = beg - excl;
+  if (SSSP) s_du[tid] = du;
+  __syncthreads();
+  for (int eb = 0; eb < total; eb += bd) {
+    const int e = eb + tid;
+    bool want = false;
+    int u = 0;
+    if (e < total) {
+      int lo = 0, hi = bd - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_end[mid] > e) hi = mid; else lo = mid + 1;
+      }
+      const int idx = s_beg[lo] + e;
+      u = g.col[idx];
+      if (!SSSP) {
+        if (s.dist[u] == kInf) {
+          uint32_t expect = kInf;
+          want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                      scope);
+        }
+      } else {
+        const uint32_t alt = s_du[lo] + (uint32_t)g.wt[idx];
+        if (alt < s.dist[u]) {
+          const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
+          if (alt < old) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+        }
+      }
+    }
+    const unsigned long long mask = __ballot(want);
+    if (mask) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)mask) - 1;
+      int base = 0;
+      if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
+      base = __shfl(base, leader, 64);
+      if (want) out_q[base + __popcll(mask & ((1ull << lane) - 1ull))] = u;
+    }
+  }
+  __syncthreads();
+}
+
+// One level of frontier relaxation over q[level & 1] by a team of nteam
+// workgroups. Static mode: chunks of bd entries round-robin over the team.
+// Dynamic mode (s.chunk > 0): workgroups claim s.chunk entries at a time from
+// a per-level counter, which balances uneven chunks.
+template <bool SSSP>
+__device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
+                                               int rank, int nteam, int* lds) {
+  const int bd = blockDim.x, tid = threadIdx.x;
   const int in_n = ld_relaxed(&s.counts[(level % 3) * kLine]);
   const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
   int32_t* out_q = (level & 1) ? s.q0 : s.q1;
   int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
   const uint32_t next = (uint32_t)level + 1u;
-
-  for (int base = rank * bd; base < in_n; base += nteam * bd) {
-    const int i = base + tid;
-    int beg = 0, deg = 0;
-    uint32_t du = 0;
-    if (i < in_n) {
-      const int v = in_q[i];
-      beg = g.row_ptr[v];
-      deg = g.row_ptr[v + 1] - beg;
-      if (SSSP) du = s.dist[v];
+  if (s.chunk > 0 && s.claims && nteam > 1) {
+    int* s_claim = lds + 3 * bd + 17;
+    int* claim = &s.claims[(level % 3) * kLine];
+    while (true) {
+      if (tid == 0) *s_claim = add_relaxed(claim, s.chunk);
+      __syncthreads();
+      const int c0 = *s_claim;
+      __syncthreads();
+      if (c0 >= in_n) break;
+      const int c1 = min(c0 + s.chunk, in_n);
+      for (int a = c0; a < c1; a += bd) frontier_batch<SSSP, false>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds);
     }
-    int total;
-    const int excl = wg_exclusive_scan(deg, s_wave, &total);
-    s_end[tid] = excl + deg;
-    s_beg[tid] = beg - excl;
-    if (SSSP) s_du[tid] = du;
-    __syncthreads();
-    for (int eb = 0; eb < total; eb += bd) {
-      const int e = eb + tid;
-      bool want = false;
-      int u = 0;
-      if (e < total) {
-        int lo = 0, hi = bd - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_end[mid] > e) hi = mid; else lo = mid + 1;
-        }
-        const int idx = s_beg[lo] + e;
-        u = g.col[idx];
-        if (!SSSP) {
-          if (s.dist[u] == kInf) {
-            uint32_t expect = kInf;
-            want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED,
-                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        } else {
-          const uint32_t alt = s_du[lo] + (uint32_t)g.wt[idx];
-          if (alt < s.dist[u]) {
-            const uint32_t old =
-                __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (alt < old)
-              want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < next;
-          }
-        }
-      }
-      wave_push(want, u, out_q, out_cnt);
-    }
-    __syncthreads();
+    if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
+  } else {
+    for (int a = rank * bd; a < in_n; a += nteam * bd)
+      frontier_batch<SSSP, false>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds);
   }
   if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
 }

@@ -133,6 +133,13 @@ int wgm_reset_ctl(wgm_runtime* rt, int im_exit, int handshake) {
   return WGM_OK;
 }
 
+// Enqueue the control reset on any stream (used inside hipGraph capture).
+int wgm_enqueue_reset_ctl(wgm_runtime* rt, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_ctl, dim3(64), dim3(256), 0, st, rt->ctl, rt->max_groups, rt->max_grid);
+  HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
 int wgm_occupancy(wgm_runtime* rt, int kernel, int wg_size, int* per_cu) {
   if (kernel < 0 || kernel >= WGM_K_COUNT) return fail(rt, WGM_ERR_INVALID, "unknown kernel id %d", kernel);
   int rc = wgm_check_wg(rt, wg_size);

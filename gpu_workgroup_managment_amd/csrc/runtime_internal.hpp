@@ -54,9 +54,13 @@ bool wgm_wait_cell(const int* cell, int value, bool equal, double timeout_s);
 // defined by the app unit (apps.hip): true / WGM_OK when `kernel` is one of its kernels
 bool wgm_app_kernel_info(int kernel, KernelInfo* out);
 int wgm_app_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
-// defined by the scheduler / batched units
+int wgm_enqueue_reset_ctl(wgm_runtime* rt, hipStream_t st);
+// defined by the batched unit (batched.hip), which forwards to the scheduler unit
 bool wgm_extra_kernel_info(int kernel, KernelInfo* out);
 int wgm_discover_extra(wgm_runtime* rt, int kernel, int grid, int wg_size);
+// defined by the scheduler unit (sched.hip)
+bool wgm_sched_kernel_info(int kernel, KernelInfo* out);
+int wgm_sched_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
 void wgm_runtime_release_extras(wgm_runtime* rt);
 
 #define WGM_HIPCHK(rt, call)                                    \

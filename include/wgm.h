@@ -178,6 +178,50 @@ int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
 int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations,
                  const wgm_launch* lc, float* rank_out, wgm_run_stats* st);
 
+/* ------------------------------------------------------- batched graphs
+ * Many BFS / SSSP queries in one persistent launch that pulls work from a
+ * device queue (no counterpart in the reference, which runs one graph per
+ * process: src/host_skeleton/skel.cpp:250-385). */
+typedef struct wgm_batch wgm_batch;
+
+typedef struct wgm_batch_config {
+  int32_t wg_size;     /* threads per workgroup, 64..1024 */
+  int32_t wgs_per_cu;  /* resident workgroups per CU requested (capped by discovery) */
+  int32_t chunk;       /* frontier entries a team workgroup claims at a time */
+  int32_t team_edges;  /* edges per team workgroup: team = ceil(m / team_edges) */
+  int32_t max_team;    /* largest team */
+  int32_t use_graph;   /* 1: capture the launch in a hipGraph and replay it */
+} wgm_batch_config;
+
+typedef struct wgm_batch_stats {
+  double wall_ms;          /* HIP event time around the whole batch */
+  double device_ms;        /* device clock, first workgroup start to last exit */
+  double gteps;            /* edges_scanned / wall time */
+  int64_t edges_scanned;   /* sum over graphs of out-degrees of reached vertices */
+  int32_t participating;   /* discovered workgroups (persistent launch) */
+  int32_t launches;        /* kernel launches that served the queries */
+  int32_t grid;            /* launch grid of the persistent kernel */
+  int32_t reserved;
+} wgm_batch_stats;
+
+/* Upload `count` graphs (host CSR; weights needed for SSSP queries) into
+ * pooled HBM arrays. queries[i] is WGM_QUERY_BFS / WGM_QUERY_SSSP; sources
+ * may be NULL (vertex 0). */
+int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const int32_t* queries,
+                     const int32_t* sources, wgm_batch** out);
+int wgm_batch_destroy(wgm_batch* b);
+int64_t wgm_batch_bytes(const wgm_batch* b);
+/* One persistent launch serving every query (cfg may be NULL for defaults). */
+int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
+/* Baseline: one persistent launch per graph, back to back. */
+int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
+/* Result of query `index` of the last run (any output may be NULL). */
+int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* iterations, int64_t* edges_scanned);
+/* Grid search over wg_size {64..1024}, wgs_per_cu {1,2,4,8}, team_edges and
+ * chunk {32..4096}; returns the fastest configuration. */
+int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best,
+                       wgm_batch_stats* best_stats);
+
 #ifdef __cplusplus
 }
 #endif

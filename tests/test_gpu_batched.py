@@ -1,0 +1,67 @@
+"""GPU parity of batched persistent BFS/SSSP: every query bit-exact vs the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_workgroup_managment_amd import batched as B
+from gpu_workgroup_managment_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    items, graphs, queries = B.workload(count=48, seed=11, lo_exp=6, hi_exp=13)
+    refs = []
+    for g, q in zip(graphs, queries):
+        refs.append(oracle.bfs(g.row_ptr, g.col, 0)[0] if q == L.QUERY_BFS else oracle.sssp(g.row_ptr, g.col, g.wt, 0))
+    return graphs, queries, refs
+
+
+def _check_all(batch, graphs, refs):
+    for i, g in enumerate(graphs):
+        dist, iters, edges = batch.result(i)
+        assert np.array_equal(dist, refs[i]), f"graph {i}: {int((dist != refs[i]).sum())} differ"
+        assert edges == int(g.degrees()[refs[i] != oracle.INF].sum())
+
+
+@pytest.mark.parametrize("team_edges,chunk", [(1 << 30, 0), (512, 0), (512, 64)])
+def test_batched_persistent(rt, small_batch, team_edges, chunk):
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    st = b.run(wg_size=256, wgs_per_cu=2, team_edges=team_edges, chunk=chunk, max_team=8)
+    print(st)
+    assert st["launches"] == 1
+    _check_all(b, graphs, refs)
+    b.close()
+
+
+def test_batched_relaunch(rt, small_batch):
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    st = b.run_relaunch(wg_size=256, wgs_per_cu=2, team_edges=2048, max_team=4)
+    assert st["launches"] == len(graphs)
+    _check_all(b, graphs, refs)
+    b.close()
+
+
+def test_batched_hipgraph_replays(rt, small_batch):
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    cfg = B.config(wg_size=128, wgs_per_cu=4, team_edges=4096, max_team=4, use_graph=True)
+    for rep in range(4):
+        b.run(cfg)
+        _check_all(b, graphs, refs)
+        # eager work between replays
+        b.run_relaunch(wg_size=64, wgs_per_cu=1, team_edges=1 << 30)
+        _check_all(b, graphs, refs)
+    b.close()
+
+
+@pytest.mark.parametrize("wg", [64, 1024])
+def test_batched_wg_sizes(rt, small_batch, wg):
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    b.run(wg_size=wg, wgs_per_cu=1, team_edges=1024, max_team=16)
+    _check_all(b, graphs, refs)
+    b.close()
