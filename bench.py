@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark: batched-graph GTEPS on one MI355X.
+
+Workload: 512 graphs (n log-uniform in [2^10, 2^18], even-indexed RMAT with
+edge factor 16 truncated to n, odd-indexed floor(sqrt n)^2 grids, weights in
+[1, 255]), BFS and SSSP queries mixed, all resident in HBM. One step = one
+persistent launch (captured in a hipGraph) that serves every query from the
+device work queue. value = sum of edges scanned (out-degrees of reached
+vertices) / step time. Also reported: the same queries as one launch per
+graph, the roofline of the batched kernel and a CPU baseline (the threaded
+oracle restatement) on a bounded sample, which also checks bit-exactness.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Tuned on MI355X by wgm_batch_autotune over this workload (wg 64..1024,
+# 1..8 workgroups per CU, team granularity, chunk 32..4096).
+TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 16, max_team=16)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--relaunch-steps", type=int, default=1)
+    ap.add_argument("--autotune", action="store_true", help="search the launch shape before timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="also run a plain copy kernel of BYTES (HBM counter calibration under rocprofv3)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(graphs, queries, dists):
+    """Bytes the batch must move at minimum, summed over queries.
+
+    BFS: 8 per scanned edge (col + dist) + 20 per reached vertex (frontier
+    read, row_ptr pair, dist write, push). SSSP: 12 per scanned edge (col,
+    weight, dist) + 20 per reached vertex."""
+    import numpy as np
+    total = 0
+    for g, q, d in zip(graphs, queries, dists):
+        reached = d != 1_000_000_000
+        m_scanned = int(g.degrees()[reached].sum())
+        n_reached = int(reached.sum())
+        total += (8 if q == 0 else 12) * m_scanned + 20 * n_reached
+    return total
+
+
+def traffic_from_profiles():
+    """HBM bytes per batched launch measured with rocprofv3 PMC counters, if
+    the summary has been committed (profiles/batched_hbm_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "batched_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return int(line.split(":")[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(graphs, queries, batch, budget_s):
+    """Threaded oracle (threads play workgroups, one barrier per level) on
+    the first graphs of the batch until the time budget is spent; returns the
+    baseline entry and whether every sampled GPU result matched."""
+    import numpy as np
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    edges = 0
+    secs = 0.0
+    done = 0
+    exact = True
+    for i, (g, q) in enumerate(zip(graphs, queries)):
+        t0 = time.perf_counter()
+        if q == 0:
+            ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
+        else:
+            ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
+        secs += time.perf_counter() - t0
+        edges += int(g.degrees()[ref != oracle.INF].sum())
+        dist, _, _ = batch.result(i)
+        exact = exact and bool(np.array_equal(dist, ref))
+        done += 1
+        if secs >= budget_s:
+            break
+    gteps = edges / secs / 1e9 if secs > 0 else 0.0
+    return {
+        "value": round(gteps, 5),
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {done} of {len(graphs)} batch queries ({edges} edges scanned), "
+                  f"threaded frontier BFS/Bellman-Ford restatement, {secs:.2f} s",
+    }, exact, done
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import gpu_workgroup_managment_amd as wgm
+    from gpu_workgroup_managment_amd import batched as B
+
+    t0 = time.time()
+    items, graphs, queries = B.workload(args.count, threads=min(16, os.cpu_count() or 1))
+    gen_s = time.time() - t0
+    rt = wgm.Runtime(0)
+    copy_cal = None
+    if args.calibrate_copy:
+        ms, gbps = rt.copy_bench(args.calibrate_copy, 3)
+        copy_cal = {"bytes": args.calibrate_copy, "ms": round(ms, 4), "gbps_rw": round(gbps, 1)}
+    batch = B.Batch(rt, graphs, queries)
+    cfg_d = dict(TUNED)
+    if args.autotune:
+        best, _ = batch.autotune(1)
+        cfg_d = {k: v for k, v in best.as_dict().items() if k != "use_graph"}
+    cfg = B.config(use_graph=True, **cfg_d)
+
+    for _ in range(args.warmup):
+        batch.run(cfg)
+    # timed region: inputs already resident; each run ends with a device sync
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        st = batch.run(cfg)
+        kernel_ms.append(st["wall_ms"])
+    t_end = time.perf_counter()
+    ms_per_step = (t_end - t_start) * 1e3 / args.steps
+    edges = st["edges_scanned"]
+    gteps = edges / (ms_per_step * 1e6)
+
+    relaunch = None
+    for _ in range(args.relaunch_steps):
+        r = batch.run_relaunch(B.config(**cfg_d))
+        relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
+    # the relaunch overwrote results: run the batched launch once more for the checks below
+    batch.run(cfg)
+
+    dists = [batch.result(i)[0] for i in range(len(graphs))]
+    alg_bytes = algorithmic_bytes(graphs, queries, dists)
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+    achieved = alg_bytes / (avg_kernel_ms * 1e6)  # GB/s
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic_from_profiles(),
+    }
+    base, exact, sampled = (None, None, 0)
+    if not args.no_cpu_baseline:
+        base, exact, sampled = cpu_baseline(graphs, queries, batch, args.cpu_budget_s)
+    info = rt.device_info()
+    line = {
+        "metric": "batched-graph GTEPS & p99 task latency vs per-graph relaunch, 1 MI355X, bit-exact",
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched-{args.count}: n log-uniform [2^10, 2^18], even RMAT ef16 truncated, odd grids, "
+                        f"w in [1,255], BFS+SSSP mixed, one persistent launch (hipGraph)",
+            "graphs": args.count,
+            "vertices": int(sum(g.n for g in graphs)),
+            "edges": int(sum(g.m for g in graphs)),
+            "edges_scanned_per_step": int(edges),
+            "launch": cfg_d,
+            "participating_wgs": st["participating"],
+        },
+        "relaunch": None if relaunch is None else {
+            "gteps": round(relaunch["gteps"], 4),
+            "ms": round(relaunch["wall_ms"], 3),
+            "launches": relaunch["launches"],
+            "speedup_batched_vs_relaunch": round(gteps / relaunch["gteps"], 2) if relaunch["gteps"] else None,
+        },
+        "copy_calibration": copy_cal,
+        "bitexact_sampled": exact,
+        "bitexact_queries_checked": sampled,
+        "kernel_ms_avg": round(avg_kernel_ms, 4),
+        "algorithmic_bytes_per_step": int(alg_bytes),
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "device": info["arch"],
+        "gen_s": round(gen_s, 2),
+    }
+    print(json.dumps(line))
+    batch.close()
+    rt.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -165,6 +165,7 @@ SIGNATURES = {
     "wgm_occupancy_query_hip": (_i32, [_vp, _i32, _i32, _P(_i32), _P(_i32)]),
     "wgm_discover_occupancy": (_i32, [_vp, _i32, _i32, _i32, _P(_i32)]),
     "wgm_barrier_bench": (_i32, [_vp, _i32, _i32, _i32, _i32, _P(BarrierResult)]),
+    "wgm_copy_bench": (_i32, [_vp, _i64, _i32, _P(ctypes.c_double), _P(ctypes.c_double)]),
     "wgm_graph_create": (_i32, [_vp, _P(Csr), _vp, _P(_vp)]),
     "wgm_graph_destroy": (_i32, [_vp]),
     "wgm_graph_bytes": (_i64, [_vp]),

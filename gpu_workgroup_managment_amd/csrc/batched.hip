@@ -134,6 +134,9 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int r
   if (rank == 0 && threadIdx.x == 0) *iters_out = level;
 }
 
+// PER_GRAPH: the same body instantiated for the one-launch-per-graph
+// baseline (a distinct symbol, so profiles separate the two modes).
+template <int PER_GRAPH>
 __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
   extern __shared__ int lds[];
   __shared__ int s_job[2];
@@ -245,6 +248,7 @@ struct wgm_batch {
   wgm_batch_config jobs_cfg{};          // config and grid the device job table was built for
   int jobs_grid = -1;
   std::vector<int64_t> reached;         // per caller index, after a run
+  std::vector<int> levels;              // per caller index, after a run
   // captured launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -254,8 +258,12 @@ struct wgm_batch {
 
 namespace {
 
+// Team size from the graph's edge count: ceil(m / team_edges), capped.
+// (Sizing by edges per level instead -- m over the observed level count --
+// was measured slower on the batched workload: weighted-grid Bellman-Ford
+// runs thousands of wide levels and needs the team's parallelism.)
 int team_size(const wgm_batch_config& cfg, int64_t m) {
-  const int64_t te = cfg.team_edges > 0 ? cfg.team_edges : (1 << 18);
+  const int64_t te = cfg.team_edges > 0 ? cfg.team_edges : (1 << 16);
   int64_t k = (m + te - 1) / te;
   if (k < 1) k = 1;
   if (k > cfg.max_team) k = cfg.max_team;
@@ -293,7 +301,7 @@ int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out
   if (c.wg_size <= 0) c.wg_size = 256;
   if (c.wgs_per_cu <= 0) c.wgs_per_cu = 4;
   if (c.max_team <= 0) c.max_team = 16;
-  if (c.team_edges <= 0) c.team_edges = 1 << 18;
+  if (c.team_edges <= 0) c.team_edges = 1 << 16;
   if (c.chunk <= 0) c.chunk = c.wg_size;
   int rc = wgm_check_wg(rt, c.wg_size);
   if (rc) return rc;
@@ -351,7 +359,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   int rc = wgm_enqueue_reset_ctl(rt, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
-  hipLaunchKernelGGL(k_batched, dim3(grid), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size), st, rt->ctl, d);
+  hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size), st, rt->ctl, d);
   WGM_HIPCHK(rt, hipGetLastError());
   return WGM_OK;
 }
@@ -363,13 +371,16 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
   hipLaunchKernelGGL(k_batch_reached, dim3(std::min(b->count, 1024)), dim3(256), 0, rt->stream, d, b->d_reached);
   WGM_HIPCHK(rt, hipGetLastError());
   std::vector<unsigned long long> per_job(b->count);
+  std::vector<int> iters(b->count);
   WGM_HIPCHK(rt, hipMemcpyAsync(per_job.data(), b->d_reached, sizeof(unsigned long long) * b->count,
                                 hipMemcpyDeviceToHost, rt->stream));
+  WGM_HIPCHK(rt, hipMemcpyAsync(iters.data(), b->iters, sizeof(int) * b->count, hipMemcpyDeviceToHost, rt->stream));
   WGM_HIPCHK(rt, hipStreamSynchronize(rt->stream));
   b->reached.assign(b->count, 0);
   int64_t total = 0;
   for (int k = 0; k < b->count; k++) {
     b->reached[b->jobs[k].index] = (int64_t)per_job[k];
+    b->levels[b->jobs[k].index] = iters[k];
     total += (int64_t)per_job[k];
   }
   if (st) st->edges_scanned = total;
@@ -381,7 +392,7 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
 // Extension hooks used by runtime.hip
 bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
   if (kernel == WGM_K_BATCHED) {
-    *out = KernelInfo{(const void*)&k_batched, 1};
+    *out = KernelInfo{(const void*)&k_batched<0>, 1};
     return true;
   }
   return wgm_sched_kernel_info(kernel, out);
@@ -389,7 +400,7 @@ bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
 
 int wgm_discover_extra(wgm_runtime* rt, int kernel, int grid, int wg_size) {
   if (kernel == WGM_K_BATCHED) {
-    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(wg_size), frontier_lds_bytes(wg_size), rt->stream, rt->ctl,
+    hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(wg_size), frontier_lds_bytes(wg_size), rt->stream, rt->ctl,
                        BatchDev{});
     return WGM_OK;
   }
@@ -404,6 +415,7 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   wgm_batch* b = new wgm_batch();
   b->rt = rt;
   b->count = count;
+  b->levels.assign(count, 0);
   auto bail = [&](int rc) { wgm_batch_destroy(b); return rc; };
   for (int i = 0; i < count; i++) {
     const wgm_csr& g = graphs[i];
@@ -573,7 +585,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
     rc = wgm_enqueue_reset_ctl(rt, rt->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, rt->stream, one);
-    hipLaunchKernelGGL(k_batched, dim3(b->jobs[k].team), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size),
+    hipLaunchKernelGGL(k_batched<1>, dim3(b->jobs[k].team), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size),
                        rt->stream, rt->ctl, one);
   }
   HIPCHK(rt, hipGetLastError());
@@ -597,12 +609,7 @@ int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* i
   if (dist_out)
     HIPCHK(rt, hipMemcpy(dist_out, b->dist + b->v_off[index], sizeof(uint32_t) * b->n[index], hipMemcpyDeviceToHost));
   if (iterations) {
-    int pos = -1;
-    for (int k = 0; k < b->count; k++)
-      if (b->jobs.size() == (size_t)b->count && b->jobs[k].index == index) pos = k;
-    int it = -1;
-    if (pos >= 0) HIPCHK(rt, hipMemcpy(&it, b->iters + pos, sizeof(int), hipMemcpyDeviceToHost));
-    *iterations = it;
+    *iterations = b->reached.empty() ? -1 : b->levels[index];
   }
   if (edges_scanned) *edges_scanned = b->reached.empty() ? -1 : b->reached[index];
   return WGM_OK;
@@ -616,7 +623,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
   if (reps < 1) reps = 1;
   const int wgs[] = {64, 128, 256, 512, 1024};
   const int per_cu[] = {1, 2, 4, 8};
-  const int team_edges[] = {1 << 16, 1 << 18, 1 << 20};
+  const int team_edges[] = {1 << 14, 1 << 16, 1 << 18};
   const int chunks[] = {32, 256, 4096};
   wgm_batch_config best{};
   wgm_batch_stats bst{};
@@ -626,7 +633,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
       if (wg * pc > 2048) continue;  // CU thread capacity
       for (int te : team_edges)
         for (int ch : chunks) {
-          if (ch != 256 && te != (1 << 18)) continue;  // chunk only swept at the middle team size
+          if (ch != 256 && te != (1 << 16)) continue;  // chunk only swept at the middle granularity
           wgm_batch_config c{};
           c.wg_size = wg;
           c.wgs_per_cu = pc;

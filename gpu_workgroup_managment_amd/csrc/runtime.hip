@@ -72,6 +72,13 @@ __global__ void __launch_bounds__(1024) k_barrier_bench(Ctl c, int iterations, i
   clock_end(c, id);
 }
 
+// Plain streaming copy (16 bytes per lane per step): the calibration kernel
+// for HBM traffic counters and the achievable-bandwidth reference.
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace wgm
 
 using namespace wgm;
@@ -449,3 +456,32 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
 }
 
 }  // extern "C"
+
+extern "C" int wgm_copy_bench(wgm_runtime* rt, int64_t bytes, int32_t iterations, double* ms_per_copy, double* gbps) {
+  if (!rt || bytes < 16 || iterations < 1) return wgm_fail(rt, WGM_ERR_INVALID, "bad copy bench arguments");
+  const long long n = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  WGM_HIPCHK(rt, hipMalloc(&src, (size_t)n * 16));
+  hipError_t e = hipMalloc(&dst, (size_t)n * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    return wgm_hip_fail(rt, e, "hipMalloc(copy dst)");
+  }
+  e = hipMemsetAsync(src, 1, (size_t)n * 16, rt->stream);
+  const int grid = rt->info.num_cus * 16;
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, rt->stream, (const uint4*)src, (uint4*)dst, n);  // warm
+  if (e == hipSuccess) e = hipEventRecord(rt->ev0, rt->stream);
+  for (int i = 0; i < iterations && e == hipSuccess; i++)
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, rt->stream, (const uint4*)src, (uint4*)dst, n);
+  if (e == hipSuccess) e = hipEventRecord(rt->ev1, rt->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(rt->ev1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, rt->ev0, rt->ev1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (e != hipSuccess) return wgm_hip_fail(rt, e, "copy bench");
+  const double per = ms / iterations;
+  if (ms_per_copy) *ms_per_copy = per;
+  if (gbps) *gbps = per > 0 ? 2.0 * (double)n * 16 / (per * 1e6) : 0.0;
+  return WGM_OK;
+}

@@ -77,6 +77,12 @@ class Runtime:
         self._check(L.lib().wgm_barrier_bench(self.handle, variant, wg_size, num_wgs, iterations, ctypes.byref(r)))
         return {f: getattr(r, f) for f, _ in r._fields_}
 
+    def copy_bench(self, nbytes=1 << 30, iterations=20):
+        """Plain copy kernel: (ms per copy, GB/s counting read + write)."""
+        ms, gbps = ctypes.c_double(), ctypes.c_double()
+        self._check(L.lib().wgm_copy_bench(self.handle, nbytes, iterations, ctypes.byref(ms), ctypes.byref(gbps)))
+        return ms.value, gbps.value
+
     def graph(self, csr):
         return DeviceGraph(self, csr)
 

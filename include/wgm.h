@@ -141,6 +141,10 @@ typedef struct wgm_barrier_result {
 int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t num_wgs,
                       int32_t iterations, wgm_barrier_result* out);
 
+/* Plain device copy of `bytes` (read + write), `iterations` times: achieved
+ * GB/s counts both directions. Calibrates HBM traffic counters. */
+int wgm_copy_bench(wgm_runtime* rt, int64_t bytes, int32_t iterations, double* ms_per_copy, double* gbps);
+
 /* ------------------------------------------------ device-resident graphs */
 typedef struct wgm_graph wgm_graph;
 /* Upload a CSR (and optional uint32 priorities) to HBM. */
@@ -188,7 +192,7 @@ typedef struct wgm_batch_config {
   int32_t wg_size;     /* threads per workgroup, 64..1024 */
   int32_t wgs_per_cu;  /* resident workgroups per CU requested (capped by discovery) */
   int32_t chunk;       /* frontier entries a team workgroup claims at a time */
-  int32_t team_edges;  /* edges per team workgroup: team = ceil(m / team_edges) */
+  int32_t team_edges;  /* edges per team workgroup: team = ceil(m / team_edges), capped at max_team */
   int32_t max_team;    /* largest team */
   int32_t use_graph;   /* 1: capture the launch in a hipGraph and replay it */
 } wgm_batch_config;
