@@ -139,6 +139,25 @@ class BatchStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class NpConfig(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("dim", ctypes.c_int32), ("length", ctypes.c_int32), ("seed", ctypes.c_int32)]
+
+
+class TaskTimes(ctypes.Structure):
+    _fields_ = [("response_us", ctypes.c_double), ("execution_us", ctypes.c_double), ("ok", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class MegaStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int32) for f in ("participating", "tasks", "np_groups", "errors", "redlines", "reserved")] + [
+        (f, ctypes.c_double) for f in ("persistent_ms", "response_mean_us", "response_p50_us", "response_p99_us",
+                                       "response_max_us", "exec_mean_us", "exec_p50_us", "exec_p99_us", "exec_max_us",
+                                       "total_mean_us", "total_p50_us", "total_p99_us", "avg_persistent_groups")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -174,6 +193,17 @@ SIGNATURES = {
     "wgm_color": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_mis": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_pagerank": (_i32, [_vp, _vp, ctypes.c_float, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_mega_create": (_i32, [_vp, _P(NpConfig), _P(_vp)]),
+    "wgm_mega_destroy": (_i32, [_vp]),
+    "wgm_mega_launch": (_i32, [_vp, _i32, _i32, _i32, _P(_i32)]),
+    "wgm_mega_send_persistent": (_i32, [_vp, _vp, _i32, _i32, _i32]),
+    "wgm_mega_persistent_running": (_i32, [_vp]),
+    "wgm_mega_wait_persistent": (_i32, [_vp, ctypes.c_double, _P(ctypes.c_double)]),
+    "wgm_mega_send_task": (_i32, [_vp, _i32, _P(TaskTimes)]),
+    "wgm_mega_quit": (_i32, [_vp]),
+    "wgm_mega_get_stats": (_i32, [_vp, _P(MegaStats)]),
+    "wgm_mega_result": (_i32, [_vp, _vp, _P(_i32)]),
+    "wgm_np_run_solo": (_i32, [_vp, _i32, _i32, _i32, _P(ctypes.c_double), _P(_i32)]),
     "wgm_batch_create": (_i32, [_vp, _i32, _P(Csr), _vp, _vp, _P(_vp)]),
     "wgm_batch_destroy": (_i32, [_vp]),
     "wgm_batch_bytes": (_i64, [_vp]),

@@ -131,7 +131,7 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
   int32_t* out_q = (level & 1) ? s.q0 : s.q1;
   int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
   const uint32_t next = (uint32_t)level + 1u;
-  if (s.chunk > 0 && s.claims && nteam > 1) {
+  if (s.chunk > 0 && s.claims) {
     int* s_claim = lds + 3 * bd + 17;
     int* claim = &s.claims[(level % 3) * kLine];
     while (true) {

@@ -144,8 +144,8 @@ struct Team {
 // --------------------------------------------------------------- discovery
 // One fetch_add joins the poll; the join order gives dense ids 0..count-1.
 // The poll closes with one fetch_or once every launched workgroup has joined
-// or the quiescence window has passed since this workgroup joined; the first
-// closer publishes the count. A workgroup whose join sees the poll closed was
+// or no workgroup has joined for the quiescence window; the first closer
+// publishes the count. A workgroup whose join sees the poll closed was
 // not co-resident with the participants and leaves. Replaces the ticket-lock
 // poll with its 12 serialised lock rounds per workgroup
 // (src/scheduler_rt/rt_device/discovery.cl:26-57).
@@ -160,12 +160,20 @@ __device__ __forceinline__ int discovery_protocol(const Ctl& c, int* count_out) 
     if (!(old & kPollClosed) && (old & kPollMask) < c.max_groups) {
       id = old & kPollMask;
       c.p_group_ids[blockIdx.x] = id;
-      const unsigned long long t0 = wall_clock64();
+      unsigned long long t0 = wall_clock64();
       const int launched = (int)gridDim.x;
+      int seen = (old & kPollMask) + 1;
       while (true) {
         int v = ld_relaxed(c.poll);
         if ((v & kPollClosed) || (v & kPollMask) >= launched) break;
-        if (wall_clock64() - t0 > c.window_ticks) break;
+        // quiescence: the window restarts whenever another workgroup joins,
+        // so a slow dispatch ramp is not mistaken for the end of residency
+        if ((v & kPollMask) != seen) {
+          seen = v & kPollMask;
+          t0 = wall_clock64();
+        } else if (wall_clock64() - t0 > c.window_ticks) {
+          break;
+        }
         spin_pause();
       }
       int before = or_acq_rel(c.poll, kPollClosed);

@@ -182,6 +182,65 @@ int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
 int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations,
                  const wgm_launch* lc, float* rank_out, wgm_run_stats* st);
 
+/* ------------------------------------------ resizing scheduler (mega-kernel)
+ * One resident mega-kernel: discovered workgroup 0 runs the scheduler, the
+ * others serve a persistent task and non-persistent tasks, moving between
+ * them at resizing barriers (offer_kill / offer_fork). Host protocol of
+ * CL_Communicator (src/scheduler_rt/rt_host/cl_communicator.h:19-465) and
+ * the run modes of src/host_skeleton/skel.cpp:168-631. */
+typedef struct wgm_mega wgm_mega;
+
+#define WGM_PERSIST_BFS 0
+#define WGM_PERSIST_SSSP 1
+
+#define WGM_NP_MATMULT_I32 0  /* reference matmult: int32, djb2 hash of the diagonal */
+#define WGM_NP_MATMUL_F32 1   /* FP32 dim x dim (dim % 32 == 0), FP64 checksum check */
+#define WGM_NP_REDUCE_MIN 2   /* int32 minimum of `length` values */
+
+typedef struct wgm_np_config {
+  int32_t kind;    /* WGM_NP_* */
+  int32_t dim;     /* matrix dimension (default 256) */
+  int32_t length;  /* reduction length (default 1 << 20) */
+  int32_t seed;    /* input seed (0: default) */
+} wgm_np_config;
+
+typedef struct wgm_task_times {
+  double response_us;   /* DEVICE_TO_TASK -> DEVICE_GOT_GROUPS */
+  double execution_us;  /* DEVICE_TO_EXECUTE -> DEVICE_WAITING */
+  int32_t ok;           /* result self-check */
+  int32_t reserved;
+} wgm_task_times;
+
+typedef struct wgm_mega_stats {
+  int32_t participating, tasks, np_groups, errors, redlines, reserved;
+  double persistent_ms;
+  double response_mean_us, response_p50_us, response_p99_us, response_max_us;
+  double exec_mean_us, exec_p50_us, exec_p99_us, exec_max_us;
+  double total_mean_us, total_p50_us, total_p99_us;
+  double avg_persistent_groups;  /* mean of the 1 ms samples of executing persistent workgroups */
+} wgm_mega_stats;
+
+/* Prepare a mega-kernel context and the non-persistent task's inputs. */
+int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out);
+int wgm_mega_destroy(wgm_mega* m);
+/* Launch the resident mega-kernel (num_wgs <= 0: discovered occupancy) and
+ * wait until the scheduler is ready; *participating excludes the scheduler. */
+int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query_barrier, int32_t* participating);
+/* Start a persistent BFS / SSSP on `groups` workgroups (returns once assigned). */
+int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups);
+int wgm_mega_persistent_running(wgm_mega* m);
+int wgm_mega_wait_persistent(wgm_mega* m, double timeout_s, double* persistent_ms);
+/* One non-persistent task on `groups` workgroups, synchronously; checked. */
+int wgm_mega_send_task(wgm_mega* m, int32_t groups, wgm_task_times* out);
+/* DEVICE_TO_QUIT and wait for the kernel to finish. */
+int wgm_mega_quit(wgm_mega* m);
+int wgm_mega_get_stats(wgm_mega* m, wgm_mega_stats* out);
+/* Result of the last persistent task (after wgm_mega_quit). */
+int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
+/* The non-persistent task launched on its own, `iterations` times. */
+int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
+                    int32_t* errors);
+
 /* ------------------------------------------------------- batched graphs
  * Many BFS / SSSP queries in one persistent launch that pulls work from a
  * device queue (no counterpart in the reference, which runs one graph per
