@@ -44,6 +44,7 @@ def lib():
             "or_reduce_min": (i32, [_i32p, i64]),
             "or_bfs_threads": (i32, [i32, _i32p, _i32p, i32, _u32p, i32]),
             "or_sssp_threads": (i32, [i32, _i32p, _i32p, _i32p, i32, _u32p, i32]),
+            "or_fnv1a64": (ctypes.c_uint64, [ctypes.c_void_p, i64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -153,3 +154,12 @@ def sssp_threads(row_ptr, col, wt, src=0, threads=8):
     out = np.empty(n, np.uint32)
     _check(lib().or_sssp_threads(n, row_ptr, col, wt, src, out, threads), "or_sssp_threads")
     return out
+
+
+def fnv1a64(data):
+    """FNV-1a 64 of a bytes object or of a numpy array's raw (little-endian) bytes."""
+    if isinstance(data, (bytes, bytearray)):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        return lib().or_fnv1a64(ctypes.cast(buf, ctypes.c_void_p), len(data))
+    a = np.ascontiguousarray(data)
+    return lib().or_fnv1a64(a.ctypes.data_as(ctypes.c_void_p), a.nbytes)

@@ -476,3 +476,16 @@ int or_sssp_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, const
   if (!wt) return -1;
   return run_threads(n, row_ptr, col, wt, src, dist, threads);
 }
+
+/* ---------------------------------------------------------------- digest
+ * FNV-1a, 64-bit: offset basis 0xcbf29ce484222325, prime 0x100000001b3
+ * (Fowler/Noll/Vo, published parameters). */
+uint64_t or_fnv1a64(const void* data, int64_t len) {
+  const unsigned char* p = (const unsigned char*)data;
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (int64_t i = 0; i < len; i++) {
+    h ^= p[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}

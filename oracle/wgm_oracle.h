@@ -52,6 +52,9 @@ int or_bfs_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, int32_
 int or_sssp_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* wt, int32_t src,
                     uint32_t* dist, int32_t threads);
 
+/* FNV-1a 64-bit digest of a byte range (fixture hashes). */
+uint64_t or_fnv1a64(const void* data, int64_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,101 @@
+"""CPU: the C-ABI boundary. libwgm.so loads without a GPU, exports every
+function include/wgm.h declares, the ctypes mirror binds exactly those, and
+every ctypes struct has the C layout (sizes and field offsets compiled from
+the header with gcc). Host-only entry points run; nothing launches a kernel."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from gpu_workgroup_managment_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
+
+STRUCTS = {
+    "wgm_csr": L.Csr,
+    "wgm_batch_item": L.BatchItem,
+    "wgm_device_info": L.DeviceInfo,
+    "wgm_barrier_result": L.BarrierResult,
+    "wgm_launch": L.Launch,
+    "wgm_run_stats": L.RunStats,
+    "wgm_batch_config": L.BatchConfig,
+    "wgm_batch_stats": L.BatchStats,
+    "wgm_np_config": L.NpConfig,
+    "wgm_task_times": L.TaskTimes,
+    "wgm_mega_stats": L.MegaStats,
+}
+STRUCTS.update(getattr(L, "EXTRA_STRUCTS", {}))
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(wgm_\w+)\s*\(", text, flags=re.M):
+            if not text[m.start():m.end()].lstrip().startswith(("typedef", "#", "struct")):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_mirror_binds_exactly_the_header():
+    assert set(L.SIGNATURES) == declared_functions()
+
+
+@pytest.fixture(scope="module")
+def layout(tmp_path_factory):
+    d = tmp_path_factory.mktemp("abi")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "wgm.h"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = d / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = d / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return dict(line.rsplit(" ", 1) for line in out.splitlines())
+
+
+@pytest.mark.parametrize("cname", sorted(STRUCTS))
+def test_struct_layout_matches_c(layout, cname):
+    cls = STRUCTS[cname]
+    assert int(layout[cname]) == ctypes.sizeof(cls)
+    for f, _ in cls._fields_:
+        assert int(layout[f"{cname}.{f}"]) == getattr(cls, f).offset, f
+
+
+def test_strerror_names_every_status():
+    lib = L.lib()
+    for code, name in L.ERRORS.items():
+        s = lib.wgm_strerror(code).decode()
+        assert s and "unknown" not in s.lower(), (code, s)
+    assert lib.wgm_strerror(0).decode()
+
+
+def test_null_arguments_are_rejected_not_dereferenced():
+    lib = L.lib()
+    assert lib.wgm_destroy(None) == 0
+    assert lib.wgm_graph_destroy(None) == 0
+    assert lib.wgm_batch_destroy(None) == 0
+    assert lib.wgm_mega_destroy(None) == 0
+    assert lib.wgm_gen_grid(0, 5, None) < 0
+    out = L.Csr()
+    assert lib.wgm_gen_grid(-1, 5, ctypes.byref(out)) == -1
+    assert lib.wgm_gen_er(10, 1000, 1, ctypes.byref(out)) == -1  # more pairs than exist
+    assert lib.wgm_batch_spec(0, 5, 10, 18, None) < 0
