@@ -26,7 +26,7 @@ ERRORS = {
 }
 
 # kernel ids (WGM_K_*)
-K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED = range(9)
+K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED, K_SERVE = range(10)
 GRAPH_RMAT, GRAPH_GRID = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
 BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD = 0, 1, 2
@@ -158,6 +158,45 @@ class MegaStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("wg_size", ctypes.c_int32), ("num_wgs", ctypes.c_int32), ("max_inflight", ctypes.c_int32),
+                ("reserve", ctypes.c_int32), ("budget_bytes", ctypes.c_int64), ("np", NpConfig)]
+
+
+class ServeDone(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("graph", ctypes.c_int32), ("query", ctypes.c_int32),
+                ("src", ctypes.c_int32), ("levels", ctypes.c_int32), ("reached", ctypes.c_int64),
+                ("edges_scanned", ctypes.c_int64), ("digest", ctypes.c_uint64), ("queue_us", ctypes.c_double),
+                ("service_us", ctypes.c_double), ("device_us", ctypes.c_double), ("preempted", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class ServeLoad(ctypes.Structure):
+    _fields_ = [("rate_per_s", ctypes.c_double), ("requests", ctypes.c_int32), ("np_groups", ctypes.c_int32),
+                ("np_fraction", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int32) for f in ("queries", "tasks", "task_errors", "reserved")] + [
+        (f, ctypes.c_double) for f in ("wall_s", "throughput_rps", "gteps", "query_p50_us", "query_p99_us",
+                                       "query_mean_us", "task_p50_us", "task_p99_us", "task_mean_us",
+                                       "task_queue_p99_us", "task_response_p99_us", "task_exec_p99_us",
+                                       "max_inflight_bytes")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+EXTRA_STRUCTS = {
+    "wgm_serve_config": ServeConfig,
+    "wgm_serve_done": ServeDone,
+    "wgm_serve_load": ServeLoad,
+    "wgm_serve_report": ServeReport,
+}
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -200,10 +239,23 @@ SIGNATURES = {
     "wgm_mega_persistent_running": (_i32, [_vp]),
     "wgm_mega_wait_persistent": (_i32, [_vp, ctypes.c_double, _P(ctypes.c_double)]),
     "wgm_mega_send_task": (_i32, [_vp, _i32, _P(TaskTimes)]),
+    "wgm_mega_send_task_kind": (_i32, [_vp, _i32, _i32, _P(TaskTimes)]),
     "wgm_mega_quit": (_i32, [_vp]),
     "wgm_mega_get_stats": (_i32, [_vp, _P(MegaStats)]),
     "wgm_mega_result": (_i32, [_vp, _vp, _P(_i32)]),
     "wgm_np_run_solo": (_i32, [_vp, _i32, _i32, _i32, _P(ctypes.c_double), _P(_i32)]),
+    "wgm_serve_create": (_i32, [_vp, _i32, _P(Csr), _P(ServeConfig), _P(_vp)]),
+    "wgm_serve_destroy": (_i32, [_vp]),
+    "wgm_serve_graph_bytes": (_i64, [_vp]),
+    "wgm_serve_start": (_i32, [_vp, _P(_i32)]),
+    "wgm_serve_submit": (_i32, [_vp, _i32, _i32, _i32, _P(_i64)]),
+    "wgm_serve_poll": (_i32, [_vp, _P(ServeDone), _i32, _P(_i32)]),
+    "wgm_serve_outstanding": (_i32, [_vp]),
+    "wgm_serve_inflight_bytes": (_i64, [_vp]),
+    "wgm_serve_task": (_i32, [_vp, _i32, _i32, _P(TaskTimes)]),
+    "wgm_serve_stop": (_i32, [_vp]),
+    "wgm_dist_digest": (_u64, [_vp, _i32]),
+    "wgm_serve_run_load": (_i32, [_vp, _P(ServeLoad), _P(ServeReport), _P(ServeDone), _P(_i32)]),
     "wgm_batch_create": (_i32, [_vp, _i32, _P(Csr), _vp, _vp, _P(_vp)]),
     "wgm_batch_destroy": (_i32, [_vp]),
     "wgm_batch_bytes": (_i64, [_vp]),

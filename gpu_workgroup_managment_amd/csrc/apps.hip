@@ -221,11 +221,11 @@ int wgm_graph_reached_edges(wgm_graph* g, int64_t* out) {
 
 bool wgm_app_kernel_info(int kernel, KernelInfo* out) {
   switch (kernel) {
-    case WGM_K_BFS: *out = KernelInfo{(const void*)&k_frontier<false>, 1}; return true;
-    case WGM_K_SSSP: *out = KernelInfo{(const void*)&k_frontier<true>, 1}; return true;
-    case WGM_K_COLOR: *out = KernelInfo{(const void*)&k_color, 0}; return true;
-    case WGM_K_MIS: *out = KernelInfo{(const void*)&k_mis, 0}; return true;
-    case WGM_K_PAGERANK: *out = KernelInfo{(const void*)&k_pagerank, 0}; return true;
+    case WGM_K_BFS: *out = KernelInfo{(const void*)&k_frontier<false>, wgm_frontier_lds}; return true;
+    case WGM_K_SSSP: *out = KernelInfo{(const void*)&k_frontier<true>, wgm_frontier_lds}; return true;
+    case WGM_K_COLOR: *out = KernelInfo{(const void*)&k_color, nullptr}; return true;
+    case WGM_K_MIS: *out = KernelInfo{(const void*)&k_mis, nullptr}; return true;
+    case WGM_K_PAGERANK: *out = KernelInfo{(const void*)&k_pagerank, nullptr}; return true;
     default: return false;
   }
 }

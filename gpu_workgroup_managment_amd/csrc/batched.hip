@@ -392,7 +392,7 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
 // Extension hooks used by runtime.hip
 bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
   if (kernel == WGM_K_BATCHED) {
-    *out = KernelInfo{(const void*)&k_batched<0>, 1};
+    *out = KernelInfo{(const void*)&k_batched<0>, wgm_frontier_lds};
     return true;
   }
   return wgm_sched_kernel_info(kernel, out);

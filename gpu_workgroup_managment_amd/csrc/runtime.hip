@@ -109,7 +109,7 @@ const KernelInfo& wgm_kernel_info(int kernel) {
   static KernelInfo table[WGM_K_COUNT];
   static bool init = false;
   if (!init) {
-    for (int k = 0; k < WGM_K_COUNT; k++) table[k] = KernelInfo{(const void*)&k_barrier_bench, 0};
+    for (int k = 0; k < WGM_K_COUNT; k++) table[k] = KernelInfo{(const void*)&k_barrier_bench, nullptr};
     for (int k = 0; k < WGM_K_COUNT; k++) {
       KernelInfo ki{};
       if (wgm_app_kernel_info(k, &ki) || wgm_extra_kernel_info(k, &ki)) table[k] = ki;
@@ -120,7 +120,8 @@ const KernelInfo& wgm_kernel_info(int kernel) {
 }
 
 size_t wgm_kernel_lds(int kernel, int wg_size) {
-  return wgm_kernel_info(kernel).frontier_lds ? (size_t)frontier_lds_bytes(wg_size) : 0;
+  const KernelInfo& ki = wgm_kernel_info(kernel);
+  return ki.lds ? (size_t)ki.lds(wg_size) : 0;
 }
 
 int wgm_check_wg(wgm_runtime* rt, int wg_size) {

@@ -37,8 +37,10 @@ struct wgm_runtime {
 
 struct KernelInfo {
   const void* fn;
-  int frontier_lds;  // 1: dynamic LDS of frontier_lds_bytes(wg_size)
+  int (*lds)(int wg_size);  // dynamic LDS bytes at a workgroup size (nullptr: none)
 };
+
+inline int wgm_frontier_lds(int wg) { return wgm::frontier_lds_bytes(wg); }
 
 const KernelInfo& wgm_kernel_info(int kernel);
 size_t wgm_kernel_lds(int kernel, int wg_size);
@@ -58,9 +60,12 @@ int wgm_enqueue_reset_ctl(wgm_runtime* rt, hipStream_t st);
 // defined by the batched unit (batched.hip), which forwards to the scheduler unit
 bool wgm_extra_kernel_info(int kernel, KernelInfo* out);
 int wgm_discover_extra(wgm_runtime* rt, int kernel, int grid, int wg_size);
-// defined by the scheduler unit (sched.hip)
+// defined by the scheduler unit (sched.hip), which forwards to the server unit
 bool wgm_sched_kernel_info(int kernel, KernelInfo* out);
 int wgm_sched_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
+// defined by the query-server unit (serve.hip)
+bool wgm_serve_kernel_info(int kernel, KernelInfo* out);
+int wgm_serve_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
 void wgm_runtime_release_extras(wgm_runtime* rt);
 
 #define WGM_HIPCHK(rt, call)                                    \

@@ -18,132 +18,11 @@ using namespace wgm;
 
 namespace wgm {
 
-__device__ __forceinline__ int pow2_ceil(int x) {
-  int p = 1;
-  while (p < x) p <<= 1;
-  return p;
-}
-
-// ================================================== non-persistent tasks
-// matmult (src/non_persistent_kernels/matmult/device/matmult.cl:26-67):
-// int32 C = A * B, elements strided over the task's workgroups; the last
-// workgroup to finish hashes the diagonal (djb2) into the host cell.
-__device__ void np_matmult_i32(const NPDev& t, int vid, int nwg) {
-  const int dim = t.dim;
-  const int c_size = dim * dim;
-  for (int i = vid * blockDim.x + threadIdx.x; i < c_size; i += nwg * blockDim.x) {
-    const int r = i / dim, col = i % dim;
-    unsigned acc = 0;
-    for (int j = 0; j < dim; j++) acc += (unsigned)t.A[r * dim + j] * (unsigned)t.B[j * dim + col];
-    t.C[i] = (int)acc;
-  }
-}
-
-// FP32 square matmul on 32x32 output tiles staged through LDS; each thread
-// owns a 2x2 block of its tile (256 threads cover 32x32), accumulating in FP32.
-__device__ void np_matmul_f32(const NPDev& t, int vid, int nwg, float* lds) {
-  const int dim = t.dim;
-  const int tiles = dim / 32;
-  float* As = lds;              // [32][33]
-  float* Bs = lds + 32 * 33;    // [32][33]
-  for (int tile = vid; tile < tiles * tiles; tile += nwg) {
-    const int tr = tile / tiles, tc = tile % tiles;
-    float acc[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-    const int ty = (threadIdx.x / 16) * 2, tx = (threadIdx.x % 16) * 2;
-    for (int k0 = 0; k0 < dim; k0 += 32) {
-      for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
-        const int r = e / 32, cc = e % 32;
-        As[r * 33 + cc] = t.Af[(tr * 32 + r) * dim + k0 + cc];
-        Bs[r * 33 + cc] = t.Bf[(k0 + r) * dim + tc * 32 + cc];
-      }
-      __syncthreads();
-      if (threadIdx.x < 256) {
-        for (int kk = 0; kk < 32; kk++) {
-          const float a0 = As[ty * 33 + kk], a1 = As[(ty + 1) * 33 + kk];
-          const float b0 = Bs[kk * 33 + tx], b1 = Bs[kk * 33 + tx + 1];
-          acc[0][0] = __builtin_fmaf(a0, b0, acc[0][0]);
-          acc[0][1] = __builtin_fmaf(a0, b1, acc[0][1]);
-          acc[1][0] = __builtin_fmaf(a1, b0, acc[1][0]);
-          acc[1][1] = __builtin_fmaf(a1, b1, acc[1][1]);
-        }
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x < 256) {
-      for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) t.Cf[(tr * 32 + ty + a) * dim + tc * 32 + tx + b] = acc[a][b];
-    }
-  }
-}
-
-// MY_reduce (src/non_persistent_kernels/reduce/device/reduce.cl:4-41): LDS
-// tree minimum per chunk of blockDim values, then one atomic min per chunk.
-__device__ void np_reduce_min(const NPDev& t, int vid, int nwg, int* lds) {
-  for (int base = vid * blockDim.x; base < t.len; base += nwg * blockDim.x) {
-    const int i = base + threadIdx.x;
-    lds[threadIdx.x] = i < t.len ? t.buf[i] : INT32_MAX;
-    __syncthreads();
-    for (int off = pow2_ceil(blockDim.x) / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off && (int)threadIdx.x + off < (int)blockDim.x)
-        lds[threadIdx.x] = min(lds[threadIdx.x], lds[threadIdx.x + off]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) __hip_atomic_fetch_min(t.dmin, lds[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-  }
-}
-
-// Task body + completion: the workgroup that finishes last publishes the
-// task's summary (hash / minimum / checksum) to the host cell with a
-// system-scope release.
-__device__ void np_task(const NPDev& t, int vid, int nwg, int* lds) {
-  if (t.kind == WGM_NP_MATMULT_I32) np_matmult_i32(t, vid, nwg);
-  else if (t.kind == WGM_NP_MATMUL_F32) np_matmul_f32(t, vid, nwg, (float*)lds);
-  else np_reduce_min(t, vid, nwg, lds);
-  drain_vmem();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    fence_release();
-    s_last = add_acq_rel(t.counter, 1) == nwg - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  if (t.kind == WGM_NP_MATMULT_I32) {
-    if (threadIdx.x == 0) {
-      unsigned h = 5381;
-      for (int i = 0; i < t.dim; i++) h = h * 33u + (unsigned)t.C[i * t.dim + i];
-      __hip_atomic_store(t.h_result, (int)h, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-  } else if (t.kind == WGM_NP_REDUCE_MIN) {
-    if (threadIdx.x == 0)
-      __hip_atomic_store(t.h_result, ld_relaxed(t.dmin), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  } else {
-    // checksum: sum of all C in FP64
-    double* red = (double*)lds;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < t.dim * t.dim; i += blockDim.x) acc += (double)t.Cf[i];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = pow2_ceil(blockDim.x) / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off && (int)threadIdx.x + off < (int)blockDim.x) red[threadIdx.x] += red[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      unsigned long long bits;
-      __builtin_memcpy(&bits, &red[0], 8);
-      __hip_atomic_store(t.h_checksum, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(t.h_result, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-}
-
 // Standalone launch of a non-persistent task (run_non_persistent,
 // src/host_skeleton/skel.cpp:168-221).
-__global__ void __launch_bounds__(1024) k_np_solo(NPDev t) {
+__global__ void __launch_bounds__(1024) k_np_solo(NPDev t, int kind) {
   extern __shared__ int lds[];
-  np_task(t, blockIdx.x, gridDim.x, lds);
+  np_task(t, kind, blockIdx.x, gridDim.x, lds);
 }
 
 // ================================================== persistent tasks
@@ -211,7 +90,7 @@ __global__ void __launch_bounds__(1024) k_mega(Ctl c, SchedDev s, const MegaApp*
     const int task = get_task(s, did, &r);
     if (task == TASK_QUIT) break;
     if (task == TASK_MULT) {
-      np_task(np, s.np.group_ids[did], k_num_groups(s.np), lds);
+      np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds);
       __syncthreads();
       if (threadIdx.x == 0) {
         fence_release();
@@ -252,6 +131,8 @@ __global__ void k_sched_reset(SchedDev s, int max_groups) {
     *s.np.executing = 0;
     *s.p.num_groups = 0;
     *s.p.executing = 0;
+    *s.np_kind = 0;
+    *s.serve_on = 0;
   }
 }
 
@@ -267,57 +148,19 @@ __global__ void k_np_reset(NPDev t) {
 #define fail wgm_fail
 #define HIPCHK WGM_HIPCHK
 
-namespace {
 
-uint64_t now_ns() {
-  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
-             std::chrono::steady_clock::now().time_since_epoch())
-      .count();
-}
-
-int hload(const int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM); }
-void hstore(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM); }
-
-// bounded spin on a host cell; yields like the reference's my_yield()
-bool hwait(const int* p, int v, double timeout_s) {
-  const uint64_t t0 = now_ns();
-  uint64_t spins = 0;
-  while (hload(p) != v) {
-    if ((++spins & 255) == 0) {
-      if ((double)(now_ns() - t0) * 1e-9 > timeout_s) return false;
-      std::this_thread::yield();
-    }
-  }
-  return true;
-}
-
-double pct(std::vector<double> v, double q) {
-  if (v.empty()) return 0.0;
-  std::sort(v.begin(), v.end());
-  const double pos = q * (double)(v.size() - 1);
-  const size_t lo = (size_t)std::floor(pos), hi = (size_t)std::ceil(pos);
-  return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
-}
-
-double mean(const std::vector<double>& v) {
-  if (v.empty()) return 0.0;
-  double s = 0;
-  for (double x : v) s += x;
-  return s / (double)v.size();
-}
-
-}  // namespace
+static int mega_lds(int wg) { return mega_lds_bytes(wg); }
 
 bool wgm_sched_kernel_info(int kernel, KernelInfo* out) {
   if (kernel == WGM_K_MEGA) {
-    *out = KernelInfo{(const void*)&k_mega, 1};
+    *out = KernelInfo{(const void*)&k_mega, mega_lds};
     return true;
   }
-  return false;
+  return wgm_serve_kernel_info(kernel, out);
 }
 
 int wgm_sched_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
-  if (kernel != WGM_K_MEGA) return wgm_fail(rt, WGM_ERR_INVALID, "no discovery probe for kernel %d", kernel);
+  if (kernel != WGM_K_MEGA) return wgm_serve_probe(rt, kernel, grid, wg_size);
   hipLaunchKernelGGL(k_mega, dim3(grid), dim3(wg_size), mega_lds_bytes(wg_size), rt->stream, rt->ctl, SchedDev{},
                      (const MegaApp*)nullptr, NPDev{});
   return WGM_OK;
@@ -326,99 +169,97 @@ int wgm_sched_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
 void wgm_runtime_release_extras(wgm_runtime* rt) { (void)rt; }
 
 // ---------------------------------------------------------------- wgm_mega
+// init_non_persistent_app for every kind at once: all buffers stay resident
+// and the kind is chosen per task.
 int mega_np_setup(wgm_mega* m, const wgm_np_config* cfg) {
   wgm_runtime* rt = m->rt;
   m->np_cfg = *cfg;
+  if (cfg->kind < WGM_NP_MATMULT_I32 || cfg->kind > WGM_NP_REDUCE_MIN)
+    return fail(rt, WGM_ERR_INVALID, "unknown non-persistent task kind %d", cfg->kind);
+  m->np_kind = cfg->kind;
   NPDev& t = m->np;
   t = NPDev{};
-  t.kind = cfg->kind;
   t.dim = cfg->dim > 0 ? cfg->dim : 256;
   t.len = cfg->length > 0 ? cfg->length : (1 << 20);
-  if (t.kind == WGM_NP_MATMUL_F32 && t.dim % 32 != 0) return fail(rt, WGM_ERR_INVALID, "FP32 matmul needs dim % 32 == 0");
+  if (t.dim % 32 != 0) return fail(rt, WGM_ERR_INVALID, "matmul dim must be a multiple of 32");
   HIPCHK(rt, hipMalloc(&m->np_dev, 4 * kLine * sizeof(int)));
   t.counter = (int*)m->np_dev;
   t.dmin = (int*)m->np_dev + kLine;
   t.h_result = m->h_np_result_dev;
   t.h_checksum = (unsigned long long*)(m->h_np_checksum_dev);
   const size_t nn = (size_t)t.dim * t.dim;
-  if (t.kind == WGM_NP_MATMULT_I32) {
-    // middle-square digit fill (src/non_persistent_kernels/matmult/matmult.h:38-50), seed 1234
-    std::vector<int32_t> A(nn), B(nn), C(nn);
-    int32_t seed = cfg->seed ? cfg->seed : 1234;
-    auto fill = [&](std::vector<int32_t>& M) {
-      for (size_t i = 0; i < nn; i++) {
-        seed = (int32_t)((uint32_t)seed * (uint32_t)seed);
-        seed = (seed / 1000) % 1000000;
-        M[i] = seed % 10;
-      }
-    };
-    fill(A);
-    fill(B);
-    // expected diagonal hash (the self-check of matmult.h:144-149,190-192)
-    uint32_t h = 5381;
-    for (int i = 0; i < t.dim; i++) {
-      uint32_t acc = 0;
-      for (int k = 0; k < t.dim; k++) acc += (uint32_t)A[(size_t)i * t.dim + k] * (uint32_t)B[(size_t)k * t.dim + i];
-      h = h * 33u + acc;
+  // matmult: middle-square digit fill (src/non_persistent_kernels/matmult/matmult.h:38-50),
+  // A then B from one stream, seed 1234
+  std::vector<int32_t> A(nn), B(nn);
+  int32_t seed = 1234;
+  auto fill = [&](std::vector<int32_t>& M) {
+    for (size_t i = 0; i < nn; i++) {
+      seed = (int32_t)((uint32_t)seed * (uint32_t)seed);
+      seed = (seed / 1000) % 1000000;
+      M[i] = seed % 10;
     }
-    m->np_expect_i = (int32_t)h;
-    HIPCHK(rt, hipMalloc(&m->np_a, nn * 4));
-    HIPCHK(rt, hipMalloc(&m->np_b, nn * 4));
-    HIPCHK(rt, hipMalloc(&m->np_c, nn * 4));
-    HIPCHK(rt, hipMemcpy(m->np_a, A.data(), nn * 4, hipMemcpyHostToDevice));
-    HIPCHK(rt, hipMemcpy(m->np_b, B.data(), nn * 4, hipMemcpyHostToDevice));
-    t.A = (const int*)m->np_a;
-    t.B = (const int*)m->np_b;
-    t.C = (int*)m->np_c;
-  } else if (t.kind == WGM_NP_MATMUL_F32) {
-    // A, B uniform in [-1, 1) from splitmix64 (seed cfg->seed)
-    std::vector<float> A(nn), B(nn);
-    uint64_t st = cfg->seed ? (uint64_t)cfg->seed : 6;
-    for (size_t i = 0; i < nn; i++) A[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
-    for (size_t i = 0; i < nn; i++) B[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
-    // expected checksum sum(C) = sum_k (colsum A)_k (rowsum B)_k in FP64
-    double sum = 0.0;
-    for (int k = 0; k < t.dim; k++) {
-      double ca = 0.0, rb = 0.0;
-      for (int i = 0; i < t.dim; i++) ca += (double)A[(size_t)i * t.dim + k];
-      for (int j = 0; j < t.dim; j++) rb += (double)B[(size_t)k * t.dim + j];
-      sum += ca * rb;
-    }
-    m->np_expect_f = sum;
-    HIPCHK(rt, hipMalloc(&m->np_a, nn * 4));
-    HIPCHK(rt, hipMalloc(&m->np_b, nn * 4));
-    HIPCHK(rt, hipMalloc(&m->np_c, nn * 4));
-    HIPCHK(rt, hipMemcpy(m->np_a, A.data(), nn * 4, hipMemcpyHostToDevice));
-    HIPCHK(rt, hipMemcpy(m->np_b, B.data(), nn * 4, hipMemcpyHostToDevice));
-    t.Af = (const float*)m->np_a;
-    t.Bf = (const float*)m->np_b;
-    t.Cf = (float*)m->np_c;
-  } else if (t.kind == WGM_NP_REDUCE_MIN) {
-    // values in [1, 2^31 - 1] from splitmix64 (the reference uses rand() + 1,
-    // src/non_persistent_kernels/reduce/reduce.h:18-35)
-    std::vector<int32_t> v(t.len);
-    uint64_t st = cfg->seed ? (uint64_t)cfg->seed : 7;
-    int32_t mn = INT32_MAX;
-    for (int i = 0; i < t.len; i++) {
-      v[i] = 1 + (int32_t)((wgm_splitmix64_next(&st) >> 33) % 2147483646ULL);
-      mn = std::min(mn, v[i]);
-    }
-    m->np_expect_i = mn;
-    HIPCHK(rt, hipMalloc(&m->np_a, (size_t)t.len * 4));
-    HIPCHK(rt, hipMemcpy(m->np_a, v.data(), (size_t)t.len * 4, hipMemcpyHostToDevice));
-    t.buf = (const int*)m->np_a;
-  } else {
-    return fail(rt, WGM_ERR_INVALID, "unknown non-persistent task kind %d", t.kind);
+  };
+  fill(A);
+  fill(B);
+  // expected diagonal hash (the self-check of matmult.h:144-149,190-192)
+  uint32_t h = 5381;
+  for (int i = 0; i < t.dim; i++) {
+    uint32_t acc = 0;
+    for (int k = 0; k < t.dim; k++) acc += (uint32_t)A[(size_t)i * t.dim + k] * (uint32_t)B[(size_t)k * t.dim + i];
+    h = h * 33u + acc;
   }
+  m->np_expect_hash = (int32_t)h;
+  // FP32 matmul: A, B uniform in [-1, 1) from splitmix64 (seed 6 unless given)
+  std::vector<float> Af(nn), Bf(nn);
+  uint64_t st = cfg->seed ? (uint64_t)cfg->seed : 6;
+  for (size_t i = 0; i < nn; i++) Af[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
+  for (size_t i = 0; i < nn; i++) Bf[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
+  // expected checksum sum(C) = sum_k (colsum A)_k (rowsum B)_k in FP64
+  double sum = 0.0;
+  for (int k = 0; k < t.dim; k++) {
+    double ca = 0.0, rb = 0.0;
+    for (int i = 0; i < t.dim; i++) ca += (double)Af[(size_t)i * t.dim + k];
+    for (int j = 0; j < t.dim; j++) rb += (double)Bf[(size_t)k * t.dim + j];
+    sum += ca * rb;
+  }
+  m->np_expect_sum = sum;
+  // reduction: values in [1, 2^31 - 1] from splitmix64 seed 7 (the reference
+  // uses rand() + 1, src/non_persistent_kernels/reduce/reduce.h:18-35)
+  std::vector<int32_t> v(t.len);
+  uint64_t rs = 7;
+  int32_t mn = INT32_MAX;
+  for (int i = 0; i < t.len; i++) {
+    v[i] = 1 + (int32_t)((wgm_splitmix64_next(&rs) >> 33) % 2147483646ULL);
+    mn = std::min(mn, v[i]);
+  }
+  m->np_expect_min = mn;
+  // one allocation: A B C (int32), Af Bf Cf (float), the reduction input
+  const size_t mat = ((nn * 4 + 255) / 256) * 256;
+  const size_t bytes = 6 * mat + (size_t)t.len * 4;
+  HIPCHK(rt, hipMalloc(&m->np_mem, bytes));
+  char* p = (char*)m->np_mem;
+  t.A = (const int*)p;
+  t.B = (const int*)(p + mat);
+  t.C = (int*)(p + 2 * mat);
+  t.Af = (const float*)(p + 3 * mat);
+  t.Bf = (const float*)(p + 4 * mat);
+  t.Cf = (float*)(p + 5 * mat);
+  t.buf = (const int*)(p + 6 * mat);
+  HIPCHK(rt, hipMemcpy((void*)t.A, A.data(), nn * 4, hipMemcpyHostToDevice));
+  HIPCHK(rt, hipMemcpy((void*)t.B, B.data(), nn * 4, hipMemcpyHostToDevice));
+  HIPCHK(rt, hipMemcpy((void*)t.Af, Af.data(), nn * 4, hipMemcpyHostToDevice));
+  HIPCHK(rt, hipMemcpy((void*)t.Bf, Bf.data(), nn * 4, hipMemcpyHostToDevice));
+  HIPCHK(rt, hipMemcpy((void*)t.buf, v.data(), (size_t)t.len * 4, hipMemcpyHostToDevice));
   return WGM_OK;
 }
 
 // reset_non_persistent, host part: the result cell. Inside the mega-kernel
 // the scheduler workgroup resets the device counters itself; a solo launch
 // (no mega-kernel resident) resets them with a kernel on the main stream.
-int mega_np_reset(wgm_mega* m, bool solo) {
+int mega_np_reset(wgm_mega* m, int kind, bool solo) {
   wgm_runtime* rt = m->rt;
-  hstore(m->h_np_result, m->np.kind == WGM_NP_MATMUL_F32 ? 0 : INT32_MIN);
+  hstore(m->h_np_result, kind == WGM_NP_MATMUL_F32 ? 0 : INT32_MIN);
+  hstore(m->h_task_kind, kind);
   if (solo) {
     hipLaunchKernelGGL(k_np_reset, dim3(1), dim3(64), 0, rt->stream, m->np);
     HIPCHK(rt, hipGetLastError());
@@ -427,17 +268,52 @@ int mega_np_reset(wgm_mega* m, bool solo) {
 }
 
 // check_non_persistent_task: compare the published summary
-bool mega_np_check(wgm_mega* m) {
+bool mega_np_check(wgm_mega* m, int kind) {
   const int r = hload(m->h_np_result);
-  if (m->np.kind == WGM_NP_MATMUL_F32) {
+  if (kind == WGM_NP_MATMUL_F32) {
     if (r != 1) return false;
     unsigned long long bits = __hip_atomic_load(m->h_np_checksum, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
     double got;
     std::memcpy(&got, &bits, 8);
-    const double tol = 1e-3 * std::max(1.0, std::fabs(m->np_expect_f)) + 1e-2;
-    return std::fabs(got - m->np_expect_f) <= tol;
+    const double tol = 1e-3 * std::max(1.0, std::fabs(m->np_expect_sum)) + 1e-2;
+    return std::fabs(got - m->np_expect_sum) <= tol;
   }
-  return r == m->np_expect_i;
+  return r == (kind == WGM_NP_MATMULT_I32 ? m->np_expect_hash : m->np_expect_min);
+}
+
+// Scheduler state reset on the runtime's stream (before a resident launch).
+int mega_enqueue_sched_reset(wgm_mega* m) {
+  wgm_runtime* rt = m->rt;
+  hipLaunchKernelGGL(k_sched_reset, dim3(64), dim3(256), 0, rt->stream, m->sd, rt->max_groups);
+  HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
+// The synchronous task protocol shared by the mega-kernel and the query
+// server: reset, TO_TASK, GOT_GROUPS, TO_EXECUTE, WAITING, check.
+int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out) {
+  wgm_runtime* rt = m->rt;
+  int rc = mega_np_reset(m, kind, false);
+  if (rc) return rc;
+  m->np_groups = groups;
+  hstore(m->h_task_size, groups);
+  const uint64_t r0 = now_ns();
+  hstore(m->h_flag, DEVICE_TO_TASK);
+  if (!hwait(m->h_flag, DEVICE_GOT_GROUPS, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_GOT_GROUPS");
+  const uint64_t r1 = now_ns();
+  hstore(m->h_flag, DEVICE_TO_EXECUTE);
+  if (!hwait(m->h_flag, DEVICE_WAITING, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_WAITING");
+  const uint64_t r2 = now_ns();
+  const bool ok = mega_np_check(m, kind);
+  if (!ok) m->errors++;
+  m->response_us.push_back((double)(r1 - r0) * 1e-3);
+  m->exec_us.push_back((double)(r2 - r1) * 1e-3);
+  if (out) {
+    out->response_us = (double)(r1 - r0) * 1e-3;
+    out->execution_us = (double)(r2 - r1) * 1e-3;
+    out->ok = ok ? 1 : 0;
+  }
+  return ok ? WGM_OK : fail(rt, WGM_ERR_MISMATCH, "non-persistent task result check failed");
 }
 
 extern "C" {
@@ -463,6 +339,7 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   m->h_persistent = m->h_cells + 3 * kLine;
   m->h_stop = m->h_cells + 4 * kLine;
   m->h_np_result = m->h_cells + 5 * kLine;
+  m->h_task_kind = m->h_cells + 7 * kLine;
   m->h_np_checksum = (unsigned long long*)(m->h_cells + 6 * kLine);
   m->h_np_result_dev = hd + 5 * kLine;
   m->h_np_checksum_dev = (unsigned long long*)(hd + 6 * kLine);
@@ -474,6 +351,7 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   s.participating = hd + 2 * kLine;
   s.persistent_flag = hd + 3 * kLine;
   s.stop = hd + 4 * kLine;
+  s.task_kind = hd + 7 * kLine;
   // device arena
   const int G = rt->max_groups;
   const size_t bytes = 16 * line_b + 2 * (size_t)G * line_b + (size_t)G * sizeof(RCtx) + 2 * (size_t)G * sizeof(int);
@@ -491,7 +369,9 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   s.np.executing = line();
   s.p.num_groups = line();
   s.p.executing = line();
-  p += 6 * line_b;
+  s.np_kind = line();
+  s.serve_on = line();
+  p += 4 * line_b;
   s.task_array = (int*)p;
   p += (size_t)G * line_b;
   s.flags = (int*)p;
@@ -513,7 +393,7 @@ int wgm_mega_destroy(wgm_mega* m) {
   if (!m) return WGM_OK;
   if (m->running) (void)wgm_mega_quit(m);
   if (m->monitor.joinable()) m->monitor.join();
-  void* ptrs[] = {m->d_arena, m->np_dev, m->np_a, m->np_b, m->np_c, m->claims};
+  void* ptrs[] = {m->d_arena, m->np_dev, m->np_mem, m->claims};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->h_cells) (void)hipHostFree(m->h_cells);
@@ -531,8 +411,6 @@ int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query
   int rc = wgm_resolve_grid(rt, WGM_K_MEGA, wg_size, num_wgs, &grid);
   if (rc) return rc;
   if (grid < 2) return fail(rt, WGM_ERR_INVALID, "the mega-kernel needs at least 2 workgroups");
-  if (m->np.kind == WGM_NP_MATMUL_F32 && wg_size < 256)
-    return fail(rt, WGM_ERR_INVALID, "the FP32 matmul task needs wg_size >= 256");
   m->wg_size = wg_size;
   m->sd.query_barrier = query_barrier ? 1 : 0;
   hstore(m->h_flag, DEVICE_SCHEDULER_INIT);
@@ -548,7 +426,8 @@ int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query
   m->np_groups = 0;
   rc = wgm_reset_ctl(rt, 0, 0);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sched_reset, dim3(64), dim3(256), 0, rt->stream, m->sd, rt->max_groups);
+  rc = mega_enqueue_sched_reset(m);
+  if (rc) return rc;
   HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
   hipLaunchKernelGGL(k_mega, dim3(grid), dim3(wg_size), mega_lds_bytes(wg_size), rt->stream, rt->ctl, m->sd,
                      (const MegaApp*)m->h_app_dev, m->np);
@@ -614,36 +493,22 @@ int wgm_mega_persistent_running(wgm_mega* m) { return (m && m->persist_running.l
 
 // send_task_synchronous (src/scheduler_rt/rt_host/cl_communicator.h:123-166):
 // response = TO_TASK -> GOT_GROUPS, execution = TO_EXECUTE -> WAITING.
-int wgm_mega_send_task(wgm_mega* m, int32_t groups, wgm_task_times* out) {
+int wgm_mega_send_task_kind(wgm_mega* m, int32_t kind, int32_t groups, wgm_task_times* out) {
   if (!m) return fail(nullptr, WGM_ERR_INVALID, "NULL mega");
   wgm_runtime* rt = m->rt;
   if (!m->running) return fail(rt, WGM_ERR_STATE, "mega-kernel not running");
+  if (kind < WGM_NP_MATMULT_I32 || kind > WGM_NP_REDUCE_MIN) return fail(rt, WGM_ERR_INVALID, "unknown task kind %d", kind);
   // the persistent master is never given up, so one workgroup stays with it
   const int limit = m->persist_running ? m->participating - 1 : m->participating;
   if (groups < 1 || groups > limit)
     return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "cannot send a task with %d workgroups (%d available to tasks)", groups,
                 limit);
-  int rc = mega_np_reset(m, false);
-  if (rc) return rc;
-  m->np_groups = groups;
-  hstore(m->h_task_size, groups);
-  const uint64_t r0 = now_ns();
-  hstore(m->h_flag, DEVICE_TO_TASK);
-  if (!hwait(m->h_flag, DEVICE_GOT_GROUPS, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_GOT_GROUPS");
-  const uint64_t r1 = now_ns();
-  hstore(m->h_flag, DEVICE_TO_EXECUTE);
-  if (!hwait(m->h_flag, DEVICE_WAITING, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_WAITING");
-  const uint64_t r2 = now_ns();
-  const bool ok = mega_np_check(m);
-  if (!ok) m->errors++;
-  m->response_us.push_back((double)(r1 - r0) * 1e-3);
-  m->exec_us.push_back((double)(r2 - r1) * 1e-3);
-  if (out) {
-    out->response_us = (double)(r1 - r0) * 1e-3;
-    out->execution_us = (double)(r2 - r1) * 1e-3;
-    out->ok = ok ? 1 : 0;
-  }
-  return ok ? WGM_OK : fail(rt, WGM_ERR_MISMATCH, "non-persistent task result check failed");
+  return mega_run_task(m, kind, groups, out);
+}
+
+int wgm_mega_send_task(wgm_mega* m, int32_t groups, wgm_task_times* out) {
+  if (!m) return fail(nullptr, WGM_ERR_INVALID, "NULL mega");
+  return wgm_mega_send_task_kind(m, m->np_kind, groups, out);
 }
 
 int wgm_mega_wait_persistent(wgm_mega* m, double timeout_s, double* persistent_ms) {
@@ -727,20 +592,19 @@ int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t itera
   wgm_runtime* rt = m->rt;
   int rc = wgm_check_wg(rt, wg_size);
   if (rc) return rc;
-  if (m->np.kind == WGM_NP_MATMUL_F32 && wg_size < 256) return fail(rt, WGM_ERR_INVALID, "FP32 matmul needs wg >= 256");
   if (m->running) return fail(rt, WGM_ERR_STATE, "the mega-kernel is resident; quit it first");
   double total = 0;
   int errs = 0;
   for (int i = 0; i < iterations; i++) {
-    rc = mega_np_reset(m, true);
+    rc = mega_np_reset(m, m->np_kind, true);
     if (rc) return rc;
     HIPCHK(rt, hipStreamSynchronize(rt->stream));
     const uint64_t t0 = now_ns();
-    hipLaunchKernelGGL(k_np_solo, dim3(num_wgs), dim3(wg_size), mega_lds_bytes(wg_size), rt->stream, m->np);
+    hipLaunchKernelGGL(k_np_solo, dim3(num_wgs), dim3(wg_size), mega_lds_bytes(wg_size), rt->stream, m->np, m->np_kind);
     HIPCHK(rt, hipGetLastError());
     HIPCHK(rt, hipStreamSynchronize(rt->stream));
     total += (double)(now_ns() - t0) * 1e-3;
-    if (!mega_np_check(m)) errs++;
+    if (!mega_np_check(m, m->np_kind)) errs++;
   }
   if (mean_us) *mean_us = total / iterations;
   if (errors) *errors = errs;

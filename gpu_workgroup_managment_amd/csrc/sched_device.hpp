@@ -56,7 +56,12 @@ struct SchedDev {
   KCtx p;
   int* np_counter;      // non-persistent task: finished-workgroup counter (reset per task)
   int* np_dmin;         // non-persistent task: running minimum (reset per task)
+  int* task_kind;       // host cell: kind of the next non-persistent task
+  int* np_kind;         // device line: kind of the task being executed
+  int* serve_on;        // device line: 1 while a query server is the persistent task
   int query_barrier;    // 1: __global_barrier_resize_query semantics
+  int serve;            // 1: the persistent task is a query server (any member may leave)
+  int reserve;          // query server: free workgroups kept back for non-persistent tasks
 };
 
 // ------------------------------------------------------------ pool lock
@@ -221,6 +226,20 @@ __device__ __forceinline__ void scheduler_init(const SchedDev& s, int count) {
 // Non-persistent tasks take discovered ids lpg, lpg-1, ... as virtual ids
 // 0, 1, ... (src/scheduler_rt/rt_device/scheduler_1.cl:83-105).
 __device__ __forceinline__ void assign_np(const SchedDev& s, int k, int lpg) {
+  if (s.serve) {
+    // Query server: any member may have left, so the free workgroups are not
+    // necessarily the top ids. Take the first k free mailboxes from the top;
+    // the gather guaranteed k of them (a mailbox stays TASK_WAIT until this
+    // thread assigns it).
+    int found = 0;
+    for (int i = lpg; found < k; i = (i == 1) ? lpg : i - 1) {
+      if (ld_relaxed(&s.task_array[i * kLine]) != TASK_WAIT) continue;
+      s.np.group_ids[i] = found++;
+      st_release(&s.task_array[i * kLine], TASK_MULT);
+      add_relaxed(s.available, -1);
+    }
+    return;
+  }
   for (int i = 0; i < k; i++) {
     s.np.group_ids[lpg - i] = i;
     while (ld_relaxed(&s.task_array[(lpg - i) * kLine]) != TASK_WAIT) {
@@ -266,12 +285,31 @@ __device__ __forceinline__ bool await_execute(const SchedDev& s) {
   return true;
 }
 
+// Query server only (no counterpart in the reference, whose persistent task
+// regains workgroups only at its own barriers): while the host is idle, free
+// workgroups beyond `reserve` rejoin the server. The server has no global
+// barrier, so a member needs no restoration context or dense id.
+__device__ __forceinline__ void serve_fork_back(const SchedDev& s, int lpg) {
+  if (ld_relaxed(s.available) <= s.reserve || ld_relaxed(s.serve_on) == 0 || sys_ld_relaxed(s.stop) != 0) return;
+  if (!try_lock(s.pool_lock)) return;
+  for (int i = 1; i <= lpg && ld_relaxed(s.available) > s.reserve; i++) {
+    if (ld_relaxed(&s.task_array[i * kLine]) != TASK_WAIT) continue;
+    add_relaxed(s.p.executing, 1);
+    sys_add_release(s.persistent_flag, 1);
+    add_relaxed(s.available, -1);
+    st_release(&s.task_array[i * kLine], TASK_PERSIST);
+  }
+  sched_unlock(s.pool_lock);
+}
+
 // src/scheduler_rt/rt_device/cl_scheduler.cl:66-171
 __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
   while (true) {
     const int flag = host_flag(s);
+    if (s.serve && flag == DEVICE_WAITING) serve_fork_back(s, lpg);
     if (flag == DEVICE_TO_QUIT) {
       fence_acquire();
+      if (s.serve) st_relaxed(s.serve_on, 0);
       while (ld_acquire(s.np.executing) != 0) {
       }
       while (ld_acquire(s.p.executing) != 0) {
@@ -307,6 +345,7 @@ __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
       }
       // reset_non_persistent, done on the device (nothing else can run on
       // the GPU while the mega-kernel holds it)
+      st_relaxed(s.np_kind, sys_ld_relaxed(s.task_kind));
       if (s.np_counter) st_relaxed(s.np_counter, 0);
       if (s.np_dmin) st_relaxed(s.np_dmin, 0x7fffffff);
       assign_np(s, k, lpg);
@@ -330,6 +369,7 @@ __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
         continue;
       }
       assign_p(s, k);
+      if (s.serve) st_relaxed(s.serve_on, 1);
       sched_unlock(s.pool_lock);
       sys_st_release(s.scheduler_flag, DEVICE_WAITING);
     }

@@ -1,33 +1,19 @@
 // sched_internal.hpp -- mega-kernel descriptors and host-side state.
 #pragma once
 
+#include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cmath>
 #include <thread>
 #include <utility>
 #include <vector>
 
 #include "apps_internal.hpp"
+#include "np_tasks.hpp"
 #include "sched_device.hpp"
 
 namespace wgm {
-
-// Non-persistent task descriptor (fixed for the life of a wgm_mega).
-struct NPDev {
-  int kind;  // WGM_NP_*
-  int dim;
-  int len;
-  const int* A;
-  const int* B;
-  int* C;
-  const float* Af;
-  const float* Bf;
-  float* Cf;
-  const int* buf;
-  int* dmin;
-  int* counter;
-  int* h_result;                   // coherent host cell
-  unsigned long long* h_checksum;  // coherent host cell
-};
 
 // Persistent task descriptor, written by the host into coherent host memory
 // before DEVICE_TO_PERSISTENT_TASK and read by each workgroup it is given to.
@@ -50,13 +36,50 @@ __device__ __forceinline__ MegaApp read_app(const MegaApp* cell) {
 }
 static_assert(sizeof(MegaApp) % 8 == 0, "descriptor copied in 8-byte words");
 
-// Dynamic LDS of the mega-kernel: the larger of the frontier layout, the
-// FP32 matmul tiles (2 x 32 x 33 floats) and the FP64 checksum reduction.
+// Dynamic LDS of the mega-kernel: the larger of the frontier layout and the
+// non-persistent tasks' tiles / reductions.
 __host__ __device__ constexpr int mega_lds_bytes(int bd) {
-  return frontier_lds_bytes(bd) > 2 * 32 * 33 * 4
-             ? (frontier_lds_bytes(bd) > bd * 8 ? frontier_lds_bytes(bd) : bd * 8)
-             : (2 * 32 * 33 * 4 > bd * 8 ? 2 * 32 * 33 * 4 : bd * 8);
+  return frontier_lds_bytes(bd) > np_lds_bytes(bd) ? frontier_lds_bytes(bd) : np_lds_bytes(bd);
 }
+
+// ----------------------------------------------- host protocol helpers
+inline uint64_t now_ns() {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+inline int hload(const int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM); }
+inline void hstore(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM); }
+
+// bounded spin on a host cell; yields like the reference's my_yield()
+inline bool hwait(const int* p, int v, double timeout_s) {
+  const uint64_t t0 = now_ns();
+  uint64_t spins = 0;
+  while (hload(p) != v) {
+    if ((++spins & 255) == 0) {
+      if ((double)(now_ns() - t0) * 1e-9 > timeout_s) return false;
+      std::this_thread::yield();
+    }
+  }
+  return true;
+}
+
+inline double pct(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const double pos = q * (double)(v.size() - 1);
+  const size_t lo = (size_t)std::floor(pos), hi = (size_t)std::ceil(pos);
+  return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+}
+
+inline double mean(const std::vector<double>& v) {
+  if (v.empty()) return 0.0;
+  double s = 0;
+  for (double x : v) s += x;
+  return s / (double)v.size();
+}
+
 
 }  // namespace wgm
 
@@ -65,13 +88,13 @@ struct wgm_mega {
   wgm::SchedDev sd{};
   wgm::NPDev np{};
   wgm_np_config np_cfg{};
-  int32_t np_expect_i = 0;
-  double np_expect_f = 0.0;
+  int np_kind = 0;               // kind of wgm_mega_send_task
+  int32_t np_expect_hash = 0;    // matmult djb2 diagonal hash
+  int32_t np_expect_min = 0;     // reduction minimum
+  double np_expect_sum = 0.0;    // FP32 matmul checksum (FP64)
   void* d_arena = nullptr;
   void* np_dev = nullptr;
-  void* np_a = nullptr;
-  void* np_b = nullptr;
-  void* np_c = nullptr;
+  void* np_mem = nullptr;        // all task buffers
   void* claims = nullptr;
   int* h_cells = nullptr;
   int* h_flag = nullptr;
@@ -79,6 +102,7 @@ struct wgm_mega {
   int* h_participating = nullptr;
   int* h_persistent = nullptr;
   int* h_stop = nullptr;
+  int* h_task_kind = nullptr;
   int* h_np_result = nullptr;
   unsigned long long* h_np_checksum = nullptr;
   int* h_np_result_dev = nullptr;
@@ -102,3 +126,7 @@ struct wgm_mega {
   int errors = 0;
   int redlines = 0;
 };
+
+// shared with the query server (serve.hip)
+int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out);
+int mega_enqueue_sched_reset(wgm_mega* m);

@@ -1,0 +1,916 @@
+// serve.hip -- the query server: a resident kernel whose persistent task
+// serves BFS / SSSP requests on resident graphs while non-persistent tasks
+// borrow workgroups through the resizing scheduler.
+//
+// No counterpart in the reference, whose driver pairs one persistent task
+// with a fixed-period stream of non-persistent tasks
+// (src/host_skeleton/skel.cpp:405-463). The scheduler protocol, the
+// non-persistent tasks and the frontier relaxation are the same as in the
+// mega-kernel (sched.hip) and the batched kernel (batched.hip).
+//
+// Device side. Discovered workgroup 0 runs the scheduler (wave 0) and a
+// doorbell (wave 1) that mirrors the host's ring tail and stop cells into
+// device lines, so serving workgroups never poll across the host link. A
+// serving workgroup takes, in order: a kill request (it leaves the server),
+// a preempted query to resume, a new query from the ring. One workgroup runs
+// one query, level by level, with the frontier counters in LDS; at every
+// level boundary it may yield: the level and frontier size go to device
+// memory and the query goes to the resume queue. Finished queries publish a
+// completion record (levels, reached, edges scanned, digest, device clock
+// stamps) into coherent host memory.
+//
+// Host side. Requests are admitted into slots while their state (distances,
+// stamps, two frontier queues) fits the HBM budget next to the resident
+// graphs, and are published through the ring; completions are harvested by
+// polling the slots' records.
+
+#include "sched_internal.hpp"
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <vector>
+
+using namespace wgm;
+
+namespace wgm {
+
+struct ServeGraphDev {
+  const int32_t* row;
+  const int32_t* col;
+  const int32_t* wt;
+  int n;
+  int m;
+};
+
+// Written by the host into its slot before the slot id is published.
+struct ServeReq {
+  int graph, query, src, seq;
+  long long state_off;  // 4-byte words into the state arena
+  long long pad;
+};
+static_assert(sizeof(ServeReq) == 32, "request descriptor is four 8-byte words");
+
+// Written by the device when the query in the slot finishes; seq last.
+struct ServeRec {
+  int seq, levels;
+  long long reached, scanned;
+  unsigned long long digest;
+  long long t0, t1;
+  int preempted, pad;
+  long long pad2;
+};
+static_assert(sizeof(ServeRec) == 64, "completion record: one 64-byte line per slot");
+
+struct ServeDev {
+  const ServeGraphDev* graphs;
+  const ServeReq* reqs;        // host memory, device view
+  ServeRec* recs;              // host memory, device view
+  const int* ring;             // host memory: published slot ids
+  const int* ring_tail_host;   // host cell: entries published
+  int* ring_tail;              // device line: mirror of ring_tail_host
+  int* ring_head;              // device line: entries claimed
+  int* stop_dev;               // device line: mirror of the host stop cell
+  int* quit_dev;               // device line: the scheduler has quit
+  int* resume;                 // [slots]: slot + 1, 0 when empty
+  int* resume_head;            // device line
+  int* resume_tail;            // device line
+  int* saved;                  // [slots * 2]: level, frontier size
+  long long* saved_t0;         // [slots]: device clock at the first level
+  int* saved_pre;              // [slots]: times yielded
+  uint32_t* arena;             // request state
+  int ring_cap;
+  int slots;
+};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Any serving workgroup may leave: the server has no global barrier.
+__device__ __forceinline__ bool take_kill(const SchedDev& s) {
+  int k = ld_relaxed(s.groups_to_kill);
+  while (k > 0) {
+    if (__hip_atomic_compare_exchange_strong(s.groups_to_kill, &k, k - 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void push_resume(const ServeDev& v, int slot) {
+  const int p = add_relaxed(v.resume_tail, 1);
+  st_release(&v.resume[p % v.slots], slot + 1);
+}
+
+__device__ __forceinline__ int pop_resume(const ServeDev& v) {
+  int h = ld_relaxed(v.resume_head);
+  while (h < ld_relaxed(v.resume_tail)) {
+    if (__hip_atomic_compare_exchange_strong(v.resume_head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+      int* cell = &v.resume[h % v.slots];
+      int x;
+      while ((x = ld_relaxed(cell)) == 0) {
+      }
+      st_relaxed(cell, 0);
+      fence_acquire();
+      return x - 1;
+    }
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int claim_ring(const ServeDev& v) {
+  int h = ld_relaxed(v.ring_head);
+  while (h < ld_acquire(v.ring_tail)) {
+    if (__hip_atomic_compare_exchange_strong(v.ring_head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return sys_ld_acquire(&v.ring[h % v.ring_cap]);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ ServeReq read_req(const ServeReq* cell) {
+  ServeReq r;
+  const unsigned long long* src = (const unsigned long long*)cell;
+  unsigned long long* dst = (unsigned long long*)&r;
+  for (int i = 0; i < 4; i++) dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  return r;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned lo = __shfl_xor((unsigned)x, off, 64);
+    const unsigned hi = __shfl_xor((unsigned)(x >> 32), off, 64);
+    x += ((unsigned long long)hi << 32) | lo;
+  }
+  return x;
+}
+
+// One query on this workgroup, from the top or from a saved level. Returns
+// true if the workgroup yielded (the query is in the resume queue).
+template <bool SSSP>
+__device__ bool serve_query(const SchedDev& s, const ServeDev& v, int slot, bool resume, const ServeReq& rq,
+                            int* lds, int* s_cnt, int* s_misc) {
+  const ServeGraphDev gd = v.graphs[rq.graph];
+  GraphDev g{};
+  g.row_ptr = gd.row;
+  g.col = gd.col;
+  g.wt = gd.wt;
+  g.n = gd.n;
+  g.m = gd.m;
+  FrontierDev f{};
+  f.dist = v.arena + rq.state_off;
+  f.stamp = f.dist + gd.n;
+  f.q0 = (int32_t*)(f.stamp + gd.n);
+  f.q1 = f.q0 + gd.n + 1;
+  if (!resume) {
+    for (int x = threadIdx.x; x < gd.n; x += blockDim.x) {
+      f.dist[x] = (x == rq.src) ? 0u : kInf;
+      if (SSSP) f.stamp[x] = 0u;
+    }
+    if (threadIdx.x == 0) {
+      f.q0[0] = rq.src;
+      s_cnt[0] = 1;
+      s_cnt[1] = 0;
+      s_cnt[2] = 0;
+      s_misc[0] = 0;
+      v.saved_t0[slot] = (long long)wall_clock64();
+      v.saved_pre[slot] = 0;
+    }
+  } else if (threadIdx.x == 0) {
+    const int level = v.saved[2 * slot];
+    s_cnt[level % 3] = v.saved[2 * slot + 1];
+    s_cnt[(level + 1) % 3] = 0;
+    s_cnt[(level + 2) % 3] = 0;
+    s_misc[0] = level;
+  }
+  drain_vmem();
+  __syncthreads();
+  int level = s_misc[0];
+  const int bd = blockDim.x;
+  while (true) {
+    const int in_n = s_cnt[level % 3];
+    if (in_n == 0) break;
+    const int32_t* in_q = (level & 1) ? f.q1 : f.q0;
+    int32_t* out_q = (level & 1) ? f.q0 : f.q1;
+    for (int a = 0; a < in_n; a += bd)
+      frontier_batch<SSSP, true>(g, f, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
+                                 (uint32_t)level + 1u, lds);
+    if (threadIdx.x == 0) {
+      s_cnt[(level + 2) % 3] = 0;
+      s_misc[1] = take_kill(s) ? 1 : 0;
+    }
+    drain_vmem();
+    __syncthreads();
+    level++;
+    if (s_misc[1]) {
+      // yield: the level is complete, its successor's frontier is in out_q
+      if (threadIdx.x == 0) {
+        v.saved[2 * slot] = level;
+        v.saved[2 * slot + 1] = s_cnt[level % 3];
+        v.saved_pre[slot] += 1;
+        fence_release();
+        push_resume(v, slot);
+      }
+      __syncthreads();
+      return true;
+    }
+  }
+  // completion: digest, reached vertices and scanned edges in one pass
+  fence_acquire();
+  unsigned long long dig = 0, reached = 0, scanned = 0;
+  for (int x = threadIdx.x; x < gd.n; x += blockDim.x) {
+    const uint32_t d = f.dist[x];
+    dig += mix64(((unsigned long long)(unsigned)x << 32) | d);
+    if (d != kInf) {
+      reached++;
+      scanned += (unsigned long long)(gd.row[x + 1] - gd.row[x]);
+    }
+  }
+  dig = wave_sum_u64(dig);
+  reached = wave_sum_u64(reached);
+  scanned = wave_sum_u64(scanned);
+  unsigned long long* red = (unsigned long long*)lds;
+  const int wave = threadIdx.x / 64, nw = (bd + 63) / 64;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = dig;
+    red[16 + wave] = reached;
+    red[32 + wave] = scanned;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0, c = 0;
+    for (int w = 0; w < nw; w++) {
+      a += red[w];
+      b += red[16 + w];
+      c += red[32 + w];
+    }
+    ServeRec* r = &v.recs[slot];
+    unsigned long long* rw = (unsigned long long*)r;
+    // words: [seq|levels] [reached] [scanned] [digest] [t0] [t1] [preempted|pad]
+    const unsigned long long t1 = wall_clock64();
+    __hip_atomic_store(rw + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(rw + 2, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(rw + 3, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(rw + 4, (unsigned long long)v.saved_t0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(rw + 5, t1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(rw + 6, (unsigned long long)(unsigned)v.saved_pre[slot], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+    const unsigned long long head = ((unsigned long long)(unsigned)level << 32) | (unsigned)rq.seq;
+    __hip_atomic_store(rw + 0, head, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  __syncthreads();
+  return false;
+}
+
+// The persistent task: serve until killed or stopped.
+__device__ void serve_loop(const SchedDev& s, const ServeDev& v, int* lds) {
+  __shared__ int s_cmd[2];
+  __shared__ int s_cnt[3];
+  __shared__ int s_misc[2];
+  __shared__ ServeReq s_req;
+  while (true) {
+    if (threadIdx.x == 0) {
+      int cmd = 0, slot = -1;  // 0 idle, 1 start, 2 resume, 3 leave
+      if (take_kill(s)) {
+        cmd = 3;
+      } else if ((slot = pop_resume(v)) >= 0) {
+        cmd = 2;
+      } else if ((slot = claim_ring(v)) >= 0) {
+        cmd = 1;
+      } else if (ld_relaxed(v.stop_dev)) {
+        cmd = 3;
+      }
+      if (cmd == 1 || cmd == 2) s_req = read_req(&v.reqs[slot]);
+      s_cmd[0] = cmd;
+      s_cmd[1] = slot;
+    }
+    __syncthreads();
+    const int cmd = s_cmd[0], slot = s_cmd[1];
+    const ServeReq rq = s_req;
+    __syncthreads();
+    if (cmd == 3) return;
+    if (cmd == 0) {
+      __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    const bool left = rq.query == WGM_QUERY_SSSP
+                          ? serve_query<true>(s, v, slot, cmd == 2, rq, lds, s_cnt, s_misc)
+                          : serve_query<false>(s, v, slot, cmd == 2, rq, lds, s_cnt, s_misc);
+    if (left) return;
+  }
+}
+
+// Wave 1 of the scheduler workgroup: host cells -> device lines.
+__device__ void doorbell_loop(const SchedDev& s, const ServeDev& v) {
+  int last_tail = 0, last_stop = 0;
+  while (ld_relaxed(v.quit_dev) == 0) {
+    const int t = sys_ld_acquire(v.ring_tail_host);
+    if (t != last_tail) {
+      st_release(v.ring_tail, t);
+      last_tail = t;
+    }
+    const int st = sys_ld_relaxed(s.stop);
+    if (st != last_stop) {
+      st_release(v.stop_dev, st);
+      last_stop = st;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__host__ __device__ constexpr int serve_lds_bytes(int bd) {
+  return mega_lds_bytes(bd) > 48 * 8 ? mega_lds_bytes(bd) : 48 * 8;
+}
+
+__global__ void __launch_bounds__(1024) k_serve(Ctl c, SchedDev s, ServeDev v, NPDev np) {
+  extern __shared__ int lds[];
+  int n;
+  const int did = discovery_protocol(c, &n);
+  if (did < 0 || c.im_exit) return;
+  if (did == 0) {
+    if (threadIdx.x == 0) {
+      scheduler_init(s, n);
+      scheduler_loop(s, n - 1);
+      st_release(v.quit_dev, 1);
+    } else if (threadIdx.x == 64) {
+      doorbell_loop(s, v);
+    }
+    __syncthreads();
+    return;
+  }
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      st_relaxed(&s.task_array[did * kLine], TASK_WAIT);
+      add_relaxed(s.available, 1);
+    }
+    RCtx r;
+    const int task = get_task(s, did, &r);
+    if (task == TASK_QUIT) break;
+    if (task == TASK_MULT) {
+      np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        fence_release();
+        add_relaxed(s.np.executing, -1);
+      }
+    } else if (task == TASK_PERSIST) {
+      serve_loop(s, v, lds);
+      drain_vmem();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        fence_release();
+        add_relaxed(s.p.executing, -1);
+        sys_add_release(s.persistent_flag, -1);
+      }
+    }
+  }
+}
+
+__global__ void k_serve_reset(ServeDev v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int i = t; i < v.slots; i += gridDim.x * blockDim.x) {
+    v.resume[i] = 0;
+    v.saved[2 * i] = 0;
+    v.saved[2 * i + 1] = 0;
+    v.saved_t0[i] = 0;
+    v.saved_pre[i] = 0;
+  }
+  if (t == 0) {
+    *v.ring_tail = 0;
+    *v.ring_head = 0;
+    *v.stop_dev = 0;
+    *v.quit_dev = 0;
+    *v.resume_head = 0;
+    *v.resume_tail = 0;
+  }
+}
+
+}  // namespace wgm
+
+#define fail wgm_fail
+#define HIPCHK WGM_HIPCHK
+
+namespace {
+
+uint64_t mix64_host(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+struct Pending {
+  int64_t id;
+  int graph, query, src;
+  uint64_t t_submit;
+};
+
+struct Inflight {
+  int64_t id = -1;
+  int graph = 0, query = 0, src = 0, seq = 0;
+  uint64_t t_submit = 0, t_publish = 0;
+  int64_t bytes = 0;
+};
+
+int serve_lds(int wg) { return serve_lds_bytes(wg); }
+
+}  // namespace
+
+struct wgm_serve {
+  wgm_runtime* rt = nullptr;
+  wgm_mega* mega = nullptr;
+  wgm_serve_config cfg{};
+  int count = 0;
+  std::vector<int32_t> gn;
+  std::vector<int64_t> gm;
+  std::vector<char> has_wt;
+  int64_t graph_bytes = 0;
+  int nmax = 0;
+  int64_t slot_words = 0;
+  void* d_graphs = nullptr;  // pooled CSR arrays
+  void* d_gtab = nullptr;    // ServeGraphDev[count]
+  void* d_lines = nullptr;   // lines + per-slot arrays
+  void* d_arena = nullptr;   // slots * slot_words
+  char* h_mem = nullptr;     // reqs, recs, ring, tail (coherent host memory)
+  char* h_mem_dev = nullptr;
+  ServeReq* h_reqs = nullptr;
+  ServeRec* h_recs = nullptr;
+  int* h_ring = nullptr;
+  int* h_ring_tail = nullptr;
+  ServeDev dev{};
+  int participating = 0;
+  bool started = false;
+  // host bookkeeping (submit / poll thread)
+  int64_t next_id = 0;
+  int seq = 0;
+  int tail = 0;
+  std::deque<Pending> pending;
+  std::vector<Inflight> inflight;
+  std::vector<int> free_slots;
+  int64_t inflight_bytes = 0;
+  int64_t max_inflight_bytes = 0;
+  std::deque<wgm_serve_done> done;
+  int64_t outstanding = 0;
+  double clock_hz = 1e8;
+};
+
+namespace {
+
+int64_t state_words(int n) { return (((int64_t)4 * n + 2 + 63) / 64) * 64; }
+
+// Harvest finished slots, then admit queued requests.
+void serve_progress(wgm_serve* s) {
+  const uint64_t now = now_ns();
+  for (int slot = 0; slot < (int)s->inflight.size(); slot++) {
+    Inflight& f = s->inflight[slot];
+    if (f.id < 0) continue;
+    const unsigned long long head =
+        __hip_atomic_load((unsigned long long*)&s->h_recs[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if ((int)(head & 0xffffffffu) != f.seq) continue;
+    const ServeRec& r = s->h_recs[slot];
+    wgm_serve_done d{};
+    d.id = f.id;
+    d.graph = f.graph;
+    d.query = f.query;
+    d.src = f.src;
+    d.levels = (int32_t)(head >> 32);
+    d.reached = r.reached;
+    d.edges_scanned = r.scanned;
+    d.digest = r.digest;
+    d.queue_us = (double)(f.t_publish - f.t_submit) * 1e-3;
+    d.service_us = (double)(now - f.t_publish) * 1e-3;
+    d.device_us = (double)(r.t1 - r.t0) / s->clock_hz * 1e6;
+    d.preempted = r.preempted;
+    s->done.push_back(d);
+    s->inflight_bytes -= f.bytes;
+    f.id = -1;
+    s->free_slots.push_back(slot);
+  }
+  const int64_t room = s->cfg.budget_bytes - s->graph_bytes;
+  while (!s->pending.empty() && !s->free_slots.empty()) {
+    const Pending& p = s->pending.front();
+    const int64_t bytes = state_words(s->gn[p.graph]) * 4;
+    if (s->inflight_bytes + bytes > room) break;  // admission: wait for memory to free up
+    const int slot = s->free_slots.back();
+    s->free_slots.pop_back();
+    Inflight& f = s->inflight[slot];
+    f.id = p.id;
+    f.graph = p.graph;
+    f.query = p.query;
+    f.src = p.src;
+    f.seq = ++s->seq;
+    f.t_submit = p.t_submit;
+    f.bytes = bytes;
+    s->inflight_bytes += bytes;
+    if (s->inflight_bytes > s->max_inflight_bytes) s->max_inflight_bytes = s->inflight_bytes;
+    ServeReq rq{};
+    rq.graph = p.graph;
+    rq.query = p.query;
+    rq.src = p.src;
+    rq.seq = f.seq;
+    rq.state_off = (long long)slot * s->slot_words;
+    std::memcpy((void*)&s->h_reqs[slot], &rq, sizeof(rq));
+    s->h_ring[s->tail % s->dev.ring_cap] = slot;
+    s->tail++;
+    f.t_publish = now_ns();
+    hstore(s->h_ring_tail, s->tail);
+    s->pending.pop_front();
+  }
+}
+
+}  // namespace
+
+bool wgm_serve_kernel_info(int kernel, KernelInfo* out) {
+  if (kernel == WGM_K_SERVE) {
+    *out = KernelInfo{(const void*)&k_serve, serve_lds};
+    return true;
+  }
+  return false;
+}
+
+int wgm_serve_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
+  if (kernel != WGM_K_SERVE) return wgm_fail(rt, WGM_ERR_INVALID, "no discovery probe for kernel %d", kernel);
+  hipLaunchKernelGGL(k_serve, dim3(grid), dim3(wg_size), serve_lds_bytes(wg_size), rt->stream, rt->ctl, SchedDev{},
+                     ServeDev{}, NPDev{});
+  return WGM_OK;
+}
+
+extern "C" {
+
+uint64_t wgm_dist_digest(const uint32_t* dist, int32_t n) {
+  uint64_t d = 0;
+  for (int32_t v = 0; v < n; v++) d += mix64_host(((uint64_t)(uint32_t)v << 32) | dist[v]);
+  return d;
+}
+
+int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const wgm_serve_config* cfg_in,
+                     wgm_serve** out) {
+  if (!rt || !graphs || !out || count < 1) return fail(rt, WGM_ERR_INVALID, "bad arguments");
+  wgm_serve_config cfg = cfg_in ? *cfg_in : wgm_serve_config{};
+  if (cfg.wg_size <= 0) cfg.wg_size = 1024;
+  if (cfg.max_inflight <= 0) cfg.max_inflight = 256;
+  if (cfg.reserve == 0) cfg.reserve = 16;
+  if (cfg.reserve < 0) cfg.reserve = 0;
+  if (cfg.budget_bytes <= 0) cfg.budget_bytes = (int64_t)64 << 30;
+  int rc = wgm_check_wg(rt, cfg.wg_size);
+  if (rc) return rc;
+  if (cfg.wg_size < 128) return fail(rt, WGM_ERR_INVALID, "the server needs wg_size >= 128");
+  wgm_serve* s = new wgm_serve();
+  s->rt = rt;
+  s->cfg = cfg;
+  s->count = count;
+  auto bail = [&](int code) { wgm_serve_destroy(s); return code; };
+  // graph pool: per graph row_ptr (n+1), col (m), wt (m, if present), 256-byte aligned
+  std::vector<int64_t> off_row(count), off_col(count), off_wt(count);
+  int64_t words = 0;
+  auto align = [](int64_t w) { return (w + 63) / 64 * 64; };
+  for (int i = 0; i < count; i++) {
+    const wgm_csr& g = graphs[i];
+    if (g.n < 1 || g.m < 0 || g.m > INT32_MAX || !g.row_ptr || (g.m > 0 && !g.col))
+      return bail(fail(rt, WGM_ERR_INVALID, "graph %d is malformed", i));
+    s->gn.push_back(g.n);
+    s->gm.push_back(g.m);
+    s->has_wt.push_back(g.wt != nullptr);
+    s->nmax = std::max(s->nmax, g.n);
+    off_row[i] = words;
+    words += align(g.n + 1);
+    off_col[i] = words;
+    words += align(std::max<int64_t>(g.m, 1));
+    off_wt[i] = -1;
+    if (g.wt) {
+      off_wt[i] = words;
+      words += align(std::max<int64_t>(g.m, 1));
+    }
+  }
+  s->graph_bytes = words * 4;
+  if (s->graph_bytes > cfg.budget_bytes)
+    return bail(fail(rt, WGM_ERR_BUDGET, "resident graphs (%lld bytes) exceed the budget (%lld)",
+                     (long long)s->graph_bytes, (long long)cfg.budget_bytes));
+  hipError_t e = hipMalloc(&s->d_graphs, (size_t)words * 4);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(graph pool)"));
+  std::vector<ServeGraphDev> tab(count);
+  for (int i = 0; i < count; i++) {
+    const wgm_csr& g = graphs[i];
+    int32_t* base = (int32_t*)s->d_graphs;
+    HIPCHK(rt, hipMemcpy(base + off_row[i], g.row_ptr, sizeof(int32_t) * (g.n + 1), hipMemcpyHostToDevice));
+    if (g.m > 0) HIPCHK(rt, hipMemcpy(base + off_col[i], g.col, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
+    if (g.wt && g.m > 0) HIPCHK(rt, hipMemcpy(base + off_wt[i], g.wt, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
+    tab[i] = ServeGraphDev{base + off_row[i], base + off_col[i], g.wt ? base + off_wt[i] : nullptr, g.n, (int)g.m};
+  }
+  e = hipMalloc(&s->d_gtab, sizeof(ServeGraphDev) * count);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(graph table)"));
+  HIPCHK(rt, hipMemcpy(s->d_gtab, tab.data(), sizeof(ServeGraphDev) * count, hipMemcpyHostToDevice));
+  // the scheduler cells and the non-persistent task buffers
+  rc = wgm_mega_create(rt, &cfg.np, &s->mega);
+  if (rc) return bail(rc);
+  // device lines + per-slot arrays + state arena
+  const int slots = cfg.max_inflight;
+  const size_t line_b = kLine * sizeof(int);
+  const size_t lines_bytes = 8 * line_b + (size_t)slots * (sizeof(int) * 4 + sizeof(long long));
+  e = hipMalloc(&s->d_lines, lines_bytes);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(server lines)"));
+  s->slot_words = state_words(s->nmax);
+  e = hipMalloc(&s->d_arena, (size_t)slots * s->slot_words * 4);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(request state)"));
+  // host memory: reqs, recs (64-byte stride), ring (2 * slots), tail
+  const int ring_cap = 2 * slots;
+  const size_t h_bytes = (size_t)slots * 32 + (size_t)slots * 64 + (size_t)ring_cap * 4 + line_b;
+  e = hipHostMalloc((void**)&s->h_mem, h_bytes, hipHostMallocCoherent | hipHostMallocMapped);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipHostMalloc(server cells)"));
+  std::memset(s->h_mem, 0, h_bytes);
+  e = hipHostGetDevicePointer((void**)&s->h_mem_dev, s->h_mem, 0);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipHostGetDevicePointer"));
+  s->h_reqs = (ServeReq*)s->h_mem;
+  ServeDev& v = s->dev;
+  v.graphs = (const ServeGraphDev*)s->d_gtab;
+  v.reqs = (const ServeReq*)s->h_mem_dev;
+  char* recs_h = s->h_mem + (size_t)slots * 32;
+  char* recs_d = s->h_mem_dev + (size_t)slots * 32;
+  s->h_recs = (ServeRec*)recs_h;
+  v.recs = (ServeRec*)recs_d;
+  s->h_ring = (int*)(recs_h + (size_t)slots * 64);
+  v.ring = (const int*)(recs_d + (size_t)slots * 64);
+  s->h_ring_tail = s->h_ring + ring_cap;
+  v.ring_tail_host = (const int*)(v.ring + ring_cap);
+  v.ring_cap = ring_cap;
+  v.slots = slots;
+  char* p = (char*)s->d_lines;
+  auto line = [&]() { int* r = (int*)p; p += line_b; return r; };
+  v.ring_tail = line();
+  v.ring_head = line();
+  v.stop_dev = line();
+  v.quit_dev = line();
+  v.resume_head = line();
+  v.resume_tail = line();
+  p += 2 * line_b;
+  v.saved_t0 = (long long*)p;
+  p += (size_t)slots * sizeof(long long);
+  v.resume = (int*)p;
+  p += (size_t)slots * sizeof(int);
+  v.saved = (int*)p;
+  p += (size_t)slots * 2 * sizeof(int);
+  v.saved_pre = (int*)p;
+  v.arena = (uint32_t*)s->d_arena;
+  s->inflight.assign(slots, Inflight{});
+  for (int i = slots - 1; i >= 0; i--) s->free_slots.push_back(i);
+  s->clock_hz = (double)rt->info.wall_clock_hz;
+  *out = s;
+  return WGM_OK;
+}
+
+int wgm_serve_destroy(wgm_serve* s) {
+  if (!s) return WGM_OK;
+  if (s->started) (void)wgm_serve_stop(s);
+  if (s->mega) (void)wgm_mega_destroy(s->mega);
+  void* ptrs[] = {s->d_graphs, s->d_gtab, s->d_lines, s->d_arena};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (s->h_mem) (void)hipHostFree(s->h_mem);
+  delete s;
+  return WGM_OK;
+}
+
+int64_t wgm_serve_graph_bytes(const wgm_serve* s) { return s ? s->graph_bytes : 0; }
+int32_t wgm_serve_outstanding(const wgm_serve* s) { return s ? (int32_t)s->outstanding : 0; }
+int64_t wgm_serve_inflight_bytes(const wgm_serve* s) { return s ? s->inflight_bytes : 0; }
+
+int wgm_serve_start(wgm_serve* s, int32_t* participating) {
+  if (!s) return fail(nullptr, WGM_ERR_INVALID, "NULL server");
+  wgm_runtime* rt = s->rt;
+  wgm_mega* m = s->mega;
+  if (s->started) return fail(rt, WGM_ERR_STATE, "server already started");
+  int grid = 0;
+  int rc = wgm_resolve_grid(rt, WGM_K_SERVE, s->cfg.wg_size, s->cfg.num_wgs, &grid);
+  if (rc) return rc;
+  if (grid < 2 + s->cfg.reserve) return fail(rt, WGM_ERR_INVALID, "grid %d leaves no serving workgroup", grid);
+  m->wg_size = s->cfg.wg_size;
+  m->sd.query_barrier = 0;
+  m->sd.serve = 1;
+  m->sd.reserve = s->cfg.reserve;
+  hstore(m->h_flag, DEVICE_SCHEDULER_INIT);
+  hstore(m->h_persistent, -1);
+  hstore(m->h_stop, 0);
+  hstore(m->h_participating, 0);
+  hstore(s->h_ring_tail, 0);
+  for (int i = 0; i < s->dev.slots; i++) {
+    unsigned long long* w = (unsigned long long*)&s->h_recs[i];
+    __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  s->tail = 0;
+  rc = wgm_reset_ctl(rt, 0, 0);
+  if (rc) return rc;
+  rc = mega_enqueue_sched_reset(m);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_serve_reset, dim3(16), dim3(256), 0, rt->stream, s->dev);
+  hipLaunchKernelGGL(k_serve, dim3(grid), dim3(s->cfg.wg_size), serve_lds_bytes(s->cfg.wg_size), rt->stream, rt->ctl,
+                     m->sd, s->dev, m->np);
+  HIPCHK(rt, hipGetLastError());
+  m->running = true;
+  s->started = true;
+  if (!hwait(m->h_flag, DEVICE_WAITING, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "server never reached DEVICE_WAITING");
+  m->participating = hload(m->h_participating);
+  s->participating = m->participating;
+  if (participating) *participating = s->participating;
+  const int k = s->participating - s->cfg.reserve;
+  if (k < 1) return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "%d participants leave no serving workgroup", s->participating);
+  // the server is the persistent task (send_persistent_task)
+  hstore(m->h_task_size, k);
+  hstore(m->h_flag, DEVICE_TO_PERSISTENT_TASK);
+  if (!hwait(m->h_flag, DEVICE_GOT_GROUPS, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_GOT_GROUPS");
+  hstore(m->h_flag, DEVICE_TO_EXECUTE);
+  if (!hwait(m->h_flag, DEVICE_WAITING, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_WAITING");
+  return WGM_OK;
+}
+
+static int serve_submit_at(wgm_serve* s, int32_t graph, int32_t query, int32_t src, uint64_t t_arrival, int64_t* id);
+
+int wgm_serve_submit(wgm_serve* s, int32_t graph, int32_t query, int32_t src, int64_t* id) {
+  if (!s) return fail(nullptr, WGM_ERR_INVALID, "NULL server");
+  return serve_submit_at(s, graph, query, src, now_ns(), id);
+}
+
+static int serve_submit_at(wgm_serve* s, int32_t graph, int32_t query, int32_t src, uint64_t t_arrival, int64_t* id) {
+  wgm_runtime* rt = s->rt;
+  if (graph < 0 || graph >= s->count) return fail(rt, WGM_ERR_INVALID, "graph index %d out of range", graph);
+  if (query != WGM_QUERY_BFS && query != WGM_QUERY_SSSP) return fail(rt, WGM_ERR_INVALID, "unknown query %d", query);
+  if (query == WGM_QUERY_SSSP && !s->has_wt[graph]) return fail(rt, WGM_ERR_INVALID, "graph %d has no weights", graph);
+  if (src < 0 || src >= s->gn[graph]) return fail(rt, WGM_ERR_INVALID, "bad source %d", src);
+  const int64_t bytes = state_words(s->gn[graph]) * 4;
+  if (s->graph_bytes + bytes > s->cfg.budget_bytes)
+    return fail(rt, WGM_ERR_BUDGET, "request state (%lld bytes) can never fit the budget", (long long)bytes);
+  Pending p{s->next_id++, graph, query, src, t_arrival};
+  s->pending.push_back(p);
+  s->outstanding++;
+  if (id) *id = p.id;
+  if (s->started) serve_progress(s);
+  return WGM_OK;
+}
+
+int wgm_serve_poll(wgm_serve* s, wgm_serve_done* out, int32_t max, int32_t* count) {
+  if (!s) return fail(nullptr, WGM_ERR_INVALID, "NULL server");
+  if (s->started) serve_progress(s);
+  int c = 0;
+  while (c < max && !s->done.empty()) {
+    if (out) out[c] = s->done.front();
+    s->done.pop_front();
+    s->outstanding--;
+    c++;
+  }
+  if (count) *count = c;
+  return WGM_OK;
+}
+
+int wgm_serve_task(wgm_serve* s, int32_t kind, int32_t groups, wgm_task_times* out) {
+  if (!s) return fail(nullptr, WGM_ERR_INVALID, "NULL server");
+  if (!s->started) return fail(s->rt, WGM_ERR_STATE, "server not started");
+  if (kind < WGM_NP_MATMULT_I32 || kind > WGM_NP_REDUCE_MIN) return fail(s->rt, WGM_ERR_INVALID, "unknown task kind");
+  if (groups < 1 || groups > s->participating)
+    return fail(s->rt, WGM_ERR_TOO_MANY_GROUPS, "cannot give %d of %d workgroups to a task", groups, s->participating);
+  return mega_run_task(s->mega, kind, groups, out);
+}
+
+int wgm_serve_stop(wgm_serve* s) {
+  if (!s) return fail(nullptr, WGM_ERR_INVALID, "NULL server");
+  wgm_runtime* rt = s->rt;
+  if (!s->started) return WGM_OK;
+  // drain: everything admitted finishes (bounded)
+  const uint64_t t0 = now_ns();
+  while (!s->pending.empty() || (int)s->free_slots.size() < s->dev.slots) {
+    serve_progress(s);
+    if ((double)(now_ns() - t0) * 1e-9 > rt->timeout_s) {
+      (void)wgm_mega_quit(s->mega);
+      s->started = false;
+      return fail(rt, WGM_ERR_TIMEOUT, "requests still in flight at stop");
+    }
+    std::this_thread::yield();
+  }
+  const int rc = wgm_mega_quit(s->mega);
+  s->started = false;
+  return rc;
+}
+
+// Open-loop load: Poisson arrivals, queries submitted by this thread at
+// their arrival times, non-persistent tasks run by a second thread in
+// arrival order (the scheduler protocol is synchronous).
+int wgm_serve_run_load(wgm_serve* s, const wgm_serve_load* load, wgm_serve_report* rep, wgm_serve_done* records,
+                       int32_t* nrecords) {
+  if (!s || !load || !rep) return fail(s ? s->rt : nullptr, WGM_ERR_INVALID, "NULL argument");
+  wgm_runtime* rt = s->rt;
+  if (!s->started) return fail(rt, WGM_ERR_STATE, "server not started");
+  if (load->rate_per_s <= 0 || load->requests < 1 || load->np_fraction < 0 || load->np_fraction > 1)
+    return fail(rt, WGM_ERR_INVALID, "bad load");
+  const int np_groups = load->np_groups > 0 ? load->np_groups : 16;
+  if (load->np_fraction > 0 && np_groups > s->participating)
+    return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "tasks of %d workgroups exceed %d", np_groups, s->participating);
+  struct Arrival {
+    uint64_t t;
+    int task, kind, graph, query;
+  };
+  std::vector<Arrival> arr(load->requests);
+  uint64_t st = load->seed ? load->seed : 6;
+  auto u01 = [&]() { return ((double)(wgm_splitmix64_next(&st) >> 11) + 1.0) * 0x1p-53; };  // (0, 1]
+  double t = 0.0;
+  int ntask = 0;
+  for (int i = 0; i < load->requests; i++) {
+    t += -std::log(u01()) / load->rate_per_s * 1e9;
+    Arrival a{(uint64_t)t, 0, 0, 0, 0};
+    if (u01() <= load->np_fraction) {
+      a.task = 1;
+      a.kind = (ntask++ & 1) ? WGM_NP_REDUCE_MIN : WGM_NP_MATMUL_F32;
+    } else {
+      a.graph = (int)(((unsigned __int128)wgm_splitmix64_next(&st) * (unsigned)s->count) >> 64);
+      const int r = a.graph % 4;
+      a.query = (r == 0 || r == 3 || !s->has_wt[a.graph]) ? WGM_QUERY_BFS : WGM_QUERY_SSSP;
+    }
+    arr[i] = a;
+  }
+  const uint64_t base = now_ns() + 1000000;  // first arrival 1 ms from now
+  std::vector<double> t_total, t_queue, t_resp, t_exec;
+  std::atomic<int> task_errors{0};
+  std::atomic<int> task_rc{WGM_OK};
+  std::thread tasks([&]() {
+    for (const Arrival& a : arr) {
+      if (!a.task) continue;
+      const uint64_t at = base + a.t;
+      while (now_ns() < at) {
+      }
+      const uint64_t t0 = now_ns();
+      wgm_task_times tt{};
+      const int rc = mega_run_task(s->mega, a.kind, np_groups, &tt);
+      const uint64_t t1 = now_ns();
+      if (rc == WGM_ERR_MISMATCH) task_errors++;
+      else if (rc) {
+        task_rc = rc;
+        return;
+      }
+      t_total.push_back((double)(t1 - at) * 1e-3);
+      t_queue.push_back((double)(t0 - at) * 1e-3);
+      t_resp.push_back(tt.response_us);
+      t_exec.push_back(tt.execution_us);
+    }
+  });
+  std::vector<double> q_lat;
+  int64_t scanned = 0;
+  int nrec = 0;
+  uint64_t last_done = base;
+  wgm_serve_done buf[64];
+  auto harvest = [&]() {
+    int32_t c = 0;
+    do {
+      wgm_serve_poll(s, buf, 64, &c);
+      const uint64_t now = now_ns();
+      for (int i = 0; i < c; i++) {
+        q_lat.push_back(buf[i].queue_us + buf[i].service_us);
+        scanned += buf[i].edges_scanned;
+        if (records && nrec < load->requests) records[nrec] = buf[i];
+        nrec++;
+        last_done = now;
+      }
+    } while (c == 64);
+  };
+  int rc = WGM_OK;
+  for (const Arrival& a : arr) {
+    if (a.task) continue;
+    const uint64_t at = base + a.t;
+    while (now_ns() < at) harvest();
+    rc = serve_submit_at(s, a.graph, a.query, 0, at, nullptr);
+    if (rc) break;
+  }
+  const uint64_t t_wait = now_ns();
+  while (rc == WGM_OK && s->outstanding > 0) {
+    harvest();
+    if ((double)(now_ns() - t_wait) * 1e-9 > rt->timeout_s) rc = fail(rt, WGM_ERR_TIMEOUT, "queries still outstanding");
+  }
+  tasks.join();
+  if (rc == WGM_OK && task_rc != WGM_OK) rc = task_rc;
+  const uint64_t end = std::max(last_done, now_ns());
+  wgm_serve_report r{};
+  r.queries = (int32_t)q_lat.size();
+  r.tasks = (int32_t)t_total.size();
+  r.task_errors = task_errors;
+  r.wall_s = (double)(end - base) * 1e-9;
+  r.throughput_rps = r.wall_s > 0 ? (double)(r.queries + r.tasks) / r.wall_s : 0.0;
+  r.gteps = r.wall_s > 0 ? (double)scanned / r.wall_s * 1e-9 : 0.0;
+  r.query_p50_us = pct(q_lat, 0.5);
+  r.query_p99_us = pct(q_lat, 0.99);
+  r.query_mean_us = mean(q_lat);
+  r.task_p50_us = pct(t_total, 0.5);
+  r.task_p99_us = pct(t_total, 0.99);
+  r.task_mean_us = mean(t_total);
+  r.task_queue_p99_us = pct(t_queue, 0.99);
+  r.task_response_p99_us = pct(t_resp, 0.99);
+  r.task_exec_p99_us = pct(t_exec, 0.99);
+  r.max_inflight_bytes = (double)s->max_inflight_bytes;
+  *rep = r;
+  if (nrecords) *nrecords = std::min(nrec, load->requests);
+  return rc;
+}
+
+}  // extern "C"

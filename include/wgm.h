@@ -104,7 +104,8 @@ typedef struct wgm_device_info {
 #define WGM_K_PAGERANK 6
 #define WGM_K_MEGA 7
 #define WGM_K_BATCHED 8
-#define WGM_K_COUNT 9
+#define WGM_K_SERVE 9
+#define WGM_K_COUNT 10
 
 int wgm_create(int32_t device, wgm_runtime** out);
 int wgm_destroy(wgm_runtime* rt);
@@ -230,8 +231,10 @@ int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query
 int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups);
 int wgm_mega_persistent_running(wgm_mega* m);
 int wgm_mega_wait_persistent(wgm_mega* m, double timeout_s, double* persistent_ms);
-/* One non-persistent task on `groups` workgroups, synchronously; checked. */
+/* One non-persistent task on `groups` workgroups, synchronously; checked.
+ * The kind is the one given at wgm_mega_create, or `kind` (WGM_NP_*). */
 int wgm_mega_send_task(wgm_mega* m, int32_t groups, wgm_task_times* out);
+int wgm_mega_send_task_kind(wgm_mega* m, int32_t kind, int32_t groups, wgm_task_times* out);
 /* DEVICE_TO_QUIT and wait for the kernel to finish. */
 int wgm_mega_quit(wgm_mega* m);
 int wgm_mega_get_stats(wgm_mega* m, wgm_mega_stats* out);
@@ -240,6 +243,98 @@ int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
 /* The non-persistent task launched on its own, `iterations` times. */
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
                     int32_t* errors);
+
+/* --------------------------------------------------------- query server
+ * Serving (no counterpart in the reference, whose driver sends one
+ * non-persistent task at a time beside one persistent task,
+ * src/host_skeleton/skel.cpp:405-463): the resident kernel's persistent task
+ * is a query server. Graphs stay resident in HBM; BFS / SSSP requests are
+ * admitted into request slots while their state fits the HBM budget, are
+ * published through a ring in coherent host memory and run concurrently,
+ * one workgroup per query. A query yields its workgroup at any level
+ * boundary when the scheduler needs workgroups for a non-persistent task and
+ * resumes on another; freed workgroups rejoin the server. */
+typedef struct wgm_serve wgm_serve;
+
+typedef struct wgm_serve_config {
+  int32_t wg_size;       /* default 1024 */
+  int32_t num_wgs;       /* <= 0: discovered occupancy */
+  int32_t max_inflight;  /* request slots (default 256) */
+  int32_t reserve;       /* free workgroups kept for non-persistent tasks (0: 16, < 0: none) */
+  int64_t budget_bytes;  /* HBM admission budget: graphs + in-flight state (default 64 GiB) */
+  wgm_np_config np;      /* non-persistent task inputs (every kind is resident) */
+} wgm_serve_config;
+
+/* One finished query. */
+typedef struct wgm_serve_done {
+  int64_t id;             /* from wgm_serve_submit */
+  int32_t graph, query;   /* graph index, WGM_QUERY_* */
+  int32_t src, levels;    /* source vertex, frontier levels */
+  int64_t reached;        /* vertices with a finite distance */
+  int64_t edges_scanned;  /* sum of out-degrees of reached vertices */
+  uint64_t digest;        /* wgm_dist_digest of the distance array */
+  double queue_us;        /* submit -> published to the device (admission wait) */
+  double service_us;      /* published -> completion seen by the host */
+  double device_us;       /* first level start -> end, device clock */
+  int32_t preempted;      /* times the query yielded its workgroup */
+  int32_t reserved;
+} wgm_serve_done;
+
+/* Upload `count` graphs (they count against the budget) and prepare the
+ * server; no kernel is launched yet. */
+int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const wgm_serve_config* cfg,
+                     wgm_serve** out);
+int wgm_serve_destroy(wgm_serve* s);
+/* Resident graph bytes (charged to the budget). */
+int64_t wgm_serve_graph_bytes(const wgm_serve* s);
+/* Launch the resident kernel and start the server on all workgroups but the
+ * scheduler and the reserve; *participating excludes the scheduler. */
+int wgm_serve_start(wgm_serve* s, int32_t* participating);
+/* Queue one request (returns at once). WGM_ERR_BUDGET if its state could
+ * never fit the budget next to the resident graphs. */
+int wgm_serve_submit(wgm_serve* s, int32_t graph, int32_t query, int32_t src, int64_t* id);
+/* Admit queued requests into free slots within the budget, and return up to
+ * `max` finished queries. */
+int wgm_serve_poll(wgm_serve* s, wgm_serve_done* out, int32_t max, int32_t* count);
+/* Requests submitted and not yet returned by wgm_serve_poll. */
+int32_t wgm_serve_outstanding(const wgm_serve* s);
+/* Bytes of in-flight request state currently charged to the budget. */
+int64_t wgm_serve_inflight_bytes(const wgm_serve* s);
+/* One non-persistent task (WGM_NP_*) on `groups` workgroups through the
+ * scheduler, synchronously; checked. May be called from another thread than
+ * submit/poll. */
+int wgm_serve_task(wgm_serve* s, int32_t kind, int32_t groups, wgm_task_times* out);
+/* Drain outstanding requests (bounded wait), stop the server, quit the kernel. */
+int wgm_serve_stop(wgm_serve* s);
+/* Order-independent digest of a distance array:
+ * sum over v of mix64(((uint64_t)v << 32) | dist[v]) mod 2^64, mix64 being
+ * splitmix64's output function. */
+uint64_t wgm_dist_digest(const uint32_t* dist, int32_t n);
+
+/* Open-loop load: Poisson arrivals of queries (graph uniform over the pool,
+ * BFS or SSSP as in wgm_batch_spec) and non-persistent tasks (FP32 matmul and
+ * int32 reduction alternating). */
+typedef struct wgm_serve_load {
+  double rate_per_s;     /* arrival rate, all classes */
+  int32_t requests;      /* arrivals to generate */
+  int32_t np_groups;     /* workgroups per non-persistent task */
+  double np_fraction;    /* share of arrivals that are non-persistent tasks */
+  uint64_t seed;         /* arrival stream (default 6) */
+} wgm_serve_load;
+
+typedef struct wgm_serve_report {
+  int32_t queries, tasks, task_errors, reserved;
+  double wall_s, throughput_rps, gteps;
+  double query_p50_us, query_p99_us, query_mean_us;
+  double task_p50_us, task_p99_us, task_mean_us;        /* arrival -> done */
+  double task_queue_p99_us, task_response_p99_us, task_exec_p99_us;
+  double max_inflight_bytes;
+} wgm_serve_report;
+
+/* Run one open-loop load to completion. `records` (optional) receives the
+ * finished queries (up to load->requests entries) and *nrecords their count. */
+int wgm_serve_run_load(wgm_serve* s, const wgm_serve_load* load, wgm_serve_report* rep, wgm_serve_done* records,
+                       int32_t* nrecords);
 
 /* ------------------------------------------------------- batched graphs
  * Many BFS / SSSP queries in one persistent launch that pulls work from a

@@ -99,3 +99,20 @@ def test_null_arguments_are_rejected_not_dereferenced():
     assert lib.wgm_gen_grid(-1, 5, ctypes.byref(out)) == -1
     assert lib.wgm_gen_er(10, 1000, 1, ctypes.byref(out)) == -1  # more pairs than exist
     assert lib.wgm_batch_spec(0, 5, 10, 18, None) < 0
+
+
+def test_dist_digest_host_function():
+    import numpy as np
+
+    rng = np.random.default_rng(5)
+    dist = rng.integers(0, 1 << 32, 1000, dtype=np.uint64).astype(np.uint32)
+    dist[::7] = 1_000_000_000
+    x = (np.arange(1000, dtype=np.uint64) << np.uint64(32)) | dist.astype(np.uint64)
+    z = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    want = int(z.sum(dtype=np.uint64))
+    got = L.lib().wgm_dist_digest(dist.ctypes.data_as(ctypes.c_void_p), len(dist))
+    assert got == want
+    # order independence is what lets the device sum it in any order
+    assert L.lib().wgm_dist_digest(None, 0) == 0

@@ -1,0 +1,136 @@
+"""GPU: the query server. Every query's distance array must equal the
+oracle's (checked through the order-independent digest, which covers all n
+entries, plus reached / scanned counts) whether it ran alone or in a mix
+with non-persistent tasks that force workgroups out of the server mid-query
+(preemption at level boundaries) and back; every task checks its own
+result; admission keeps in-flight state within the HBM budget."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_workgroup_managment_amd import graphs as G
+from gpu_workgroup_managment_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+os.environ.setdefault("WGM_HOST_TIMEOUT_S", "30")
+
+
+def _pool():
+    """Small graphs drawn as in the batched workload (RMAT / grid), weighted,
+    plus two long-diameter grids that keep queries in many levels."""
+    items = G.batch_spec(12, G.SEED_BATCH, 10, 13)
+    gs = G.batch_generate(items, 8)
+    for g in gs:
+        G.add_weights(g, 1, 255, 3)
+    for side in (96, 128):
+        gs.append(G.add_weights(G.grid(side, side), 1, 255, 3))
+    return gs
+
+
+@pytest.fixture(scope="module")
+def pool():
+    gs = _pool()
+    want = {}
+    for i, g in enumerate(gs):
+        b = oracle.bfs(g.row_ptr, g.col, 0)[0]
+        d = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
+        for q, dist in ((S.QUERY_BFS, b), (S.QUERY_SSSP, d)):
+            reached = dist != oracle.INF
+            want[(i, q)] = (S.dist_digest(dist), int(reached.sum()), int(g.degrees()[reached].sum()))
+    return gs, want
+
+
+def _check(recs, want):
+    assert recs
+    for r in recs:
+        dig, reached, scanned = want[(r["graph"], r["query"])]
+        assert (r["digest"], r["reached"], r["edges_scanned"]) == (dig, reached, scanned), r
+
+
+def test_serve_queries_alone(rt, pool):
+    gs, want = pool
+    with S.Server(rt, gs, S.config(wg_size=1024, max_inflight=32)) as srv:
+        p = srv.start()
+        print("participating", p, "graph bytes", srv.graph_bytes)
+        recs = []
+        for i in range(len(gs)):
+            for q in (S.QUERY_BFS, S.QUERY_SSSP):
+                srv.submit(i, q)
+                recs += srv.drain()  # one at a time: each query alone on the server
+        srv.stop()
+    _check(recs, want)
+    assert len(recs) == 2 * len(gs)
+
+
+def test_serve_mix_with_tasks_preempts_and_matches(rt, pool):
+    gs, want = pool
+    big = [len(gs) - 1, len(gs) - 2]  # the grids
+    # a small server (23 workgroups) so that queries occupy every serving workgroup
+    with S.Server(rt, gs, S.config(wg_size=1024, num_wgs=24, max_inflight=64, reserve=4)) as srv:
+        p = srv.start()
+        assert p == 23
+        recs, tasks = [], []
+        for rep in range(3):
+            for i in range(len(gs)):
+                srv.submit(i, S.QUERY_SSSP if (i + rep) % 2 else S.QUERY_BFS)
+            for i in big:
+                srv.submit(i, S.QUERY_BFS)
+                srv.submit(i, S.QUERY_SSSP)
+            # tasks wider than the reserve take workgroups from running queries
+            for k, kind in enumerate((S.NP_MATMUL_F32, S.NP_REDUCE_MIN, S.NP_MATMULT_I32)):
+                tasks.append(srv.task(kind, 8 + 6 * k))
+                recs += srv.poll()
+            recs += srv.drain()
+        srv.stop()
+    print("tasks", tasks[:3], "preempted", sum(r["preempted"] for r in recs))
+    assert all(t["ok"] for t in tasks)
+    _check(recs, want)
+    assert len(recs) == 3 * (len(gs) + 4)
+    assert sum(r["preempted"] for r in recs) > 0  # queries really yielded and resumed
+
+
+def test_serve_admission_budget(rt, pool):
+    gs, want = pool
+    probe = S.Server(rt, gs, S.config(max_inflight=8))
+    gbytes = probe.graph_bytes
+    probe.close()
+    nmax = max(g.n for g in gs)
+    state = ((4 * nmax + 2 + 63) // 64) * 64 * 4
+    budget = gbytes + 2 * state  # room for two of the largest requests at a time
+    with S.Server(rt, gs, S.config(max_inflight=32, budget_bytes=budget)) as srv:
+        srv.start()
+        peak = 0
+        for i in range(len(gs)):
+            srv.submit(i, S.QUERY_BFS)
+            peak = max(peak, srv.inflight_bytes())
+        recs = srv.drain()
+        srv.stop()
+    assert peak <= budget - gbytes
+    _check(recs, want)
+    assert len(recs) == len(gs)
+
+
+def test_serve_rejects_what_never_fits(rt, pool):
+    gs, _ = pool
+    probe = S.Server(rt, gs, S.config(max_inflight=8))
+    gbytes = probe.graph_bytes
+    probe.close()
+    with S.Server(rt, gs, S.config(max_inflight=8, budget_bytes=gbytes + 4096)) as srv:
+        with pytest.raises(Exception) as e:
+            srv.submit(len(gs) - 1, S.QUERY_BFS)
+        assert "BUDGET" in str(e.value)
+
+
+def test_serve_open_loop_load(rt, pool):
+    gs, want = pool
+    with S.Server(rt, gs, S.config(wg_size=1024, max_inflight=128, reserve=16)) as srv:
+        srv.start()
+        rep = srv.run_load(rate_per_s=4000.0, requests=400, np_fraction=0.5, np_groups=16, seed=6)
+        srv.stop()
+    recs = rep.pop("records")
+    print(rep)
+    assert rep["task_errors"] == 0
+    assert rep["queries"] + rep["tasks"] == 400
+    _check(recs, want)
