@@ -33,10 +33,81 @@ __device__ __forceinline__ int pow2_ceil(int x) {
   return p;
 }
 
-// Bytes of dynamic LDS the tasks need at a workgroup size.
+// LDS of one 256-thread slice of the MFMA matmul: an A chunk (64 rows x 16
+// k, row pitch 17) and a B chunk (16 k x 64 columns, row pitch 65), floats.
+constexpr int kMfmaSliceFloats = 64 * 17 + 16 * 65;
+
+// Bytes of dynamic LDS the tasks need at a workgroup size: the larger of the
+// matmul slices (int32 tiles: 2 x 32 x 33 words; FP32 MFMA chunks) and the
+// FP64 checksum reduction (bd doubles).
 __host__ __device__ constexpr int np_lds_bytes(int bd) {
-  return (bd >= 256 ? (bd / 256) : 1) * 2 * 32 * 33 * 4 > bd * 8 ? (bd >= 256 ? (bd / 256) : 1) * 2 * 32 * 33 * 4
-                                                                  : bd * 8;
+  return (bd >= 256 ? bd / 256 : 1) * kMfmaSliceFloats * 4 > bd * 8 ? (bd >= 256 ? bd / 256 : 1) * kMfmaSliceFloats * 4
+                                                                   : bd * 8;
+}
+
+// Checksum weights of the FP32 matmul: sum_ij C[i][j] * u(i) * v(j) catches a
+// transposed or permuted result, which a plain sum would not.
+__host__ __device__ constexpr int np_weight_u(int i) { return i % 7 + 1; }
+__host__ __device__ constexpr int np_weight_v(int j) { return j % 5 + 1; }
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 square matmul on the matrix cores. A 256-thread slice computes a 64x64
+// block of C as 2x2 wave tiles of 32x32 with v_mfma_f32_32x32x2f32, streaming
+// K in chunks of 16 through LDS with one 16-byte load of A and of B per thread. Operand layout of the
+// instruction (wave64): lane l supplies A[row l%32][k l/32] and
+// B[k l/32][col l%32]; accumulator register r of lane l holds
+// C[8(r/4) + 4(l/32) + r%4][l%32]. dim % 64 == 0.
+__device__ void np_matmul_f32_mfma(const float* A, const float* B, float* C, int dim, int vid, int nwg, float* lds) {
+  const int q = blockDim.x / 256;
+  const int slice = threadIdx.x / 256, lt = threadIdx.x % 256;
+  const int wave = lt / 64, lane = lt % 64;
+  float* As = lds + slice * kMfmaSliceFloats;
+  float* Bs = As + 64 * 17;
+  const int blocks = dim / 64;
+  const int nb = blocks * blocks;
+  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;
+  for (int base = vid * q; base < nb; base += nwg * q) {
+    const int blk = base + slice;
+    const bool live = blk < nb;
+    const int br = live ? (blk / blocks) * 64 : 0, bc = live ? (blk % blocks) * 64 : 0;
+    f32x16 acc;
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < dim; k0 += 16) {
+      if (live) {
+        const int ra = lt / 4, ca = (lt % 4) * 4;  // 64 rows x 4 float4
+        const float4 x = *(const float4*)&A[(br + ra) * dim + k0 + ca];
+        float* da = &As[ra * 17 + ca];
+        da[0] = x.x;
+        da[1] = x.y;
+        da[2] = x.z;
+        da[3] = x.w;
+        const int rb = lt / 16, cb = (lt % 16) * 4;  // 16 rows x 16 float4
+        const float4 y = *(const float4*)&B[(k0 + rb) * dim + bc + cb];
+        float* db = &Bs[rb * 65 + cb];
+        db[0] = y.x;
+        db[1] = y.y;
+        db[2] = y.z;
+        db[3] = y.w;
+      }
+      __syncthreads();
+      if (live) {
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 2) {
+          const float a = As[(wr + lane % 32) * 17 + kk + lane / 32];
+          const float b = Bs[(kk + lane / 32) * 65 + wc + lane % 32];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+    if (live) {
+      for (int r = 0; r < 16; r++) {
+        const int i = br + wr + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        C[i * dim + bc + wc + lane % 32] = acc[r];
+      }
+    }
+  }
 }
 
 // Square matmul on 32x32 output tiles staged through LDS. A workgroup of
@@ -120,7 +191,14 @@ __device__ void np_reduce_min(const NPDev& t, int vid, int nwg, int* lds) {
   const int nvec = t.len / 4;
   const int4* v4 = (const int4*)t.buf;
   const int stride = nwg * blockDim.x;
-  for (int i = vid * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+  // four independent 16-byte loads in flight per iteration
+  int i = vid * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    const int4 x0 = v4[i], x1 = v4[i + stride], x2 = v4[i + 2 * stride], x3 = v4[i + 3 * stride];
+    mn = min(mn, min(min(min(x0.x, x0.y), min(x0.z, x0.w)), min(min(x1.x, x1.y), min(x1.z, x1.w))));
+    mn = min(mn, min(min(min(x2.x, x2.y), min(x2.z, x2.w)), min(min(x3.x, x3.y), min(x3.z, x3.w))));
+  }
+  for (; i < nvec; i += stride) {
     const int4 x = v4[i];
     mn = min(mn, min(min(x.x, x.y), min(x.z, x.w)));
   }
@@ -147,7 +225,8 @@ __device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
     if (tiled) np_matmul_tiled<unsigned, int>(t.A, t.B, t.C, t.dim, vid, nwg, (unsigned*)lds);
     else np_matmul_strided<unsigned, int>(t.A, t.B, t.C, t.dim, vid, nwg);
   } else if (kind == WGM_NP_MATMUL_F32) {
-    if (tiled) np_matmul_tiled<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
+    if (tiled && t.dim % 64 == 0) np_matmul_f32_mfma(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
+    else if (tiled) np_matmul_tiled<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
     else np_matmul_strided<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg);
   } else {
     np_reduce_min(t, vid, nwg, lds);
@@ -171,10 +250,11 @@ __device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
     if (threadIdx.x == 0)
       __hip_atomic_store(t.h_result, ld_relaxed(t.dmin), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   } else {
-    // checksum: sum of all C in FP64
+    // weighted checksum in FP64: sum_ij C[i][j] u(i) v(j)
     double* red = (double*)lds;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < t.dim * t.dim; i += blockDim.x) acc += (double)t.Cf[i];
+    for (int i = threadIdx.x; i < t.dim * t.dim; i += blockDim.x)
+      acc += (double)t.Cf[i] * (double)(np_weight_u(i / t.dim) * np_weight_v(i % t.dim));
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int off = pow2_ceil(blockDim.x) / 2; off > 0; off >>= 1) {

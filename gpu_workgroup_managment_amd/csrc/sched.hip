@@ -214,15 +214,25 @@ int mega_np_setup(wgm_mega* m, const wgm_np_config* cfg) {
   uint64_t st = cfg->seed ? (uint64_t)cfg->seed : 6;
   for (size_t i = 0; i < nn; i++) Af[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
   for (size_t i = 0; i < nn; i++) Bf[i] = (float)((double)(wgm_splitmix64_next(&st) >> 11) * 0x1p-53 * 2.0 - 1.0);
-  // expected checksum sum(C) = sum_k (colsum A)_k (rowsum B)_k in FP64
-  double sum = 0.0;
+  // expected weighted checksum u^T (A B) v = sum_k (u^T A)_k (B v)_k in FP64,
+  // and its tolerance: the FP32 dot-product bound |fl(x.y) - x.y| <= K u sum|x_k y_k|
+  // summed with the same weights (doubled for any accumulation order)
+  double sum = 0.0, abs_sum = 0.0;
   for (int k = 0; k < t.dim; k++) {
-    double ca = 0.0, rb = 0.0;
-    for (int i = 0; i < t.dim; i++) ca += (double)Af[(size_t)i * t.dim + k];
-    for (int j = 0; j < t.dim; j++) rb += (double)Bf[(size_t)k * t.dim + j];
-    sum += ca * rb;
+    double ua = 0.0, bv = 0.0, uaa = 0.0, bva = 0.0;
+    for (int i = 0; i < t.dim; i++) {
+      ua += np_weight_u(i) * (double)Af[(size_t)i * t.dim + k];
+      uaa += np_weight_u(i) * std::fabs((double)Af[(size_t)i * t.dim + k]);
+    }
+    for (int j = 0; j < t.dim; j++) {
+      bv += (double)Bf[(size_t)k * t.dim + j] * np_weight_v(j);
+      bva += std::fabs((double)Bf[(size_t)k * t.dim + j]) * np_weight_v(j);
+    }
+    sum += ua * bv;
+    abs_sum += uaa * bva;
   }
   m->np_expect_sum = sum;
+  m->np_tol = 2.0 * (double)t.dim * 0x1p-24 * abs_sum + 1e-6 * std::fabs(sum);
   // reduction: values in [1, 2^31 - 1] from splitmix64 seed 7 (the reference
   // uses rand() + 1, src/non_persistent_kernels/reduce/reduce.h:18-35)
   std::vector<int32_t> v(t.len);
@@ -275,8 +285,7 @@ bool mega_np_check(wgm_mega* m, int kind) {
     unsigned long long bits = __hip_atomic_load(m->h_np_checksum, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
     double got;
     std::memcpy(&got, &bits, 8);
-    const double tol = 1e-3 * std::max(1.0, std::fabs(m->np_expect_sum)) + 1e-2;
-    return std::fabs(got - m->np_expect_sum) <= tol;
+    return std::fabs(got - m->np_expect_sum) <= m->np_tol;
   }
   return r == (kind == WGM_NP_MATMULT_I32 ? m->np_expect_hash : m->np_expect_min);
 }

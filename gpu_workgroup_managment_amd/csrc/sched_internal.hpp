@@ -91,7 +91,8 @@ struct wgm_mega {
   int np_kind = 0;               // kind of wgm_mega_send_task
   int32_t np_expect_hash = 0;    // matmult djb2 diagonal hash
   int32_t np_expect_min = 0;     // reduction minimum
-  double np_expect_sum = 0.0;    // FP32 matmul checksum (FP64)
+  double np_expect_sum = 0.0;    // FP32 matmul weighted checksum (FP64)
+  double np_tol = 0.0;           // its FP32 error bound
   void* d_arena = nullptr;
   void* np_dev = nullptr;
   void* np_mem = nullptr;        // all task buffers

@@ -15,11 +15,13 @@ pytestmark = pytest.mark.gpu
 os.environ.setdefault("WGM_HOST_TIMEOUT_S", "30")
 
 
+@pytest.mark.parametrize("wg,wgs", [(256, 16), (1024, 4), (128, 8), (1024, 1)])
 @pytest.mark.parametrize("kind", [S.NP_MATMULT_I32, S.NP_MATMUL_F32, S.NP_REDUCE_MIN])
-def test_np_solo(rt, kind):
+def test_np_solo(rt, kind, wg, wgs):
+    # FP32: MFMA path (wg % 256 == 0) and strided path (wg 128), position-weighted checksum
     with S.Mega(rt, np_kind=kind) as m:
-        us, errs = m.run_solo(wg_size=256, num_wgs=16, iterations=3)
-    print(kind, us, errs)
+        us, errs = m.run_solo(wg_size=wg, num_wgs=wgs, iterations=3)
+    print(kind, wg, wgs, us, errs)
     assert errs == 0
 
 
