@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true", help="search the launch shape before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-s", type=float, default=0.5, help="arrival window per offered load")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a plain copy kernel of BYTES (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -108,6 +110,60 @@ def cpu_baseline(graphs, queries, batch, budget_s):
     }, exact, done
 
 
+def serving_sweep(rt, graphs, queries, batch_digests, loads, duration_s, np_fraction=0.5, np_groups=16):
+    """Open-loop serving on the same resident graphs: Poisson arrivals, half
+    non-persistent tasks (FP32 256x256 matmul and 1M int32 minimum,
+    alternating), half BFS/SSSP queries. The offered load is the utilisation
+    of the synchronous task protocol, the resource tasks queue on:
+    rho = task arrival rate x mean task service time measured alone."""
+    from gpu_workgroup_managment_amd import serving as S
+    with S.Server(rt, graphs, S.config(wg_size=1024, max_inflight=256, reserve=np_groups)) as srv:
+        part = srv.start()
+        solo = []
+        for i in range(40):
+            t = srv.task(S.NP_MATMUL_F32 if i % 2 == 0 else S.NP_REDUCE_MIN, np_groups)
+            if not t["ok"]:
+                raise RuntimeError("non-persistent task failed its check")
+            solo.append(t["response_us"] + t["execution_us"])
+        s_np = sorted(solo)[len(solo) // 2] * 1e-6  # median service time, seconds
+        out = []
+        mismatched = 0
+        checked = 0
+        for rho in loads:
+            rate = rho / (np_fraction * s_np)
+            n = max(200, int(rate * duration_s))
+            rep = srv.run_load(rate_per_s=rate, requests=n, np_fraction=np_fraction, np_groups=np_groups, seed=6)
+            recs = rep.pop("records")
+            for r in recs:
+                checked += 1
+                if batch_digests[r["graph"]] != r["digest"]:
+                    mismatched += 1
+            out.append({
+                "offered_load": rho,
+                "rate_per_s": round(rate, 1),
+                "requests": n,
+                "task_p50_us": round(rep["task_p50_us"], 2),
+                "task_p99_us": round(rep["task_p99_us"], 2),
+                "task_queue_p99_us": round(rep["task_queue_p99_us"], 2),
+                "task_response_p99_us": round(rep["task_response_p99_us"], 2),
+                "task_exec_p99_us": round(rep["task_exec_p99_us"], 2),
+                "query_p50_us": round(rep["query_p50_us"], 1),
+                "query_p99_us": round(rep["query_p99_us"], 1),
+                "throughput_rps": round(rep["throughput_rps"], 1),
+                "query_gteps": round(rep["gteps"], 4),
+                "task_errors": rep["task_errors"],
+            })
+        srv.stop()
+    return {
+        "participating_wgs": part,
+        "np_groups": np_groups,
+        "np_service_us_alone": round(s_np * 1e6, 2),
+        "loads": out,
+        "queries_checked_vs_batched": checked,
+        "queries_mismatched": mismatched,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -170,6 +226,14 @@ def main():
     if not args.no_cpu_baseline:
         base, exact, sampled = cpu_baseline(graphs, queries, batch, args.cpu_budget_s)
     info = rt.device_info()
+    serving = None
+    p99_at_half = None
+    if not args.no_serving:
+        from gpu_workgroup_managment_amd import serving as S
+        digests = [S.dist_digest(d) for d in dists]
+        batch.close()
+        serving = serving_sweep(rt, graphs, queries, digests, (0.25, 0.5, 0.75), args.serving_s)
+        p99_at_half = next(x["task_p99_us"] for x in serving["loads"] if x["offered_load"] == 0.5)
     line = {
         "metric": "batched-graph GTEPS & p99 task latency vs per-graph relaunch, 1 MI355X, bit-exact",
         "value": round(gteps, 4),
@@ -204,6 +268,8 @@ def main():
         "bitexact_queries_checked": sampled,
         "kernel_ms_avg": round(avg_kernel_ms, 4),
         "algorithmic_bytes_per_step": int(alg_bytes),
+        "p99_task_latency_us_at_50pct_load": p99_at_half,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": base,
         "device": info["arch"],
