@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true", help="search the launch shape before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--suite", choices=("batched", "apps"), default="batched",
+                    help="apps: one line per standalone kernel on its workload (not the headline line)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-s", type=float, default=0.5, help="arrival window per offered load")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -164,8 +166,127 @@ def serving_sweep(rt, graphs, queries, batch_digests, loads, duration_s, np_frac
     }
 
 
+def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5):
+    """Standalone persistent kernels on their configured workloads: BFS on
+    ER 2^16 (avg degree 8), SSSP on RMAT scale 20 (ef 16, w in [1, 255]),
+    color / MIS / PageRank on the 2048x2048 grid. One JSON line per app with
+    the best launch shape, the HBM roofline of its algorithmic bytes, the
+    oracle check and the oracle's CPU time."""
+    import numpy as np
+
+    import gpu_workgroup_managment_amd as wgm
+    import oracle
+    from gpu_workgroup_managment_amd import graphs as G
+
+    rt = wgm.Runtime(0)
+    threads = min(16, os.cpu_count() or 1)
+    out = []
+
+    def timed(fn):
+        best = None
+        for wg, nw in shapes:
+            fn(wg_size=wg, num_wgs=nw)  # warm-up (and discovery cache)
+            runs = [fn(wg_size=wg, num_wgs=nw) for _ in range(reps)]
+            k = sorted(r[1]["kernel_ms"] for r in runs)[reps // 2]
+            d = sorted(r[1]["device_ms"] for r in runs)[reps // 2]
+            cand = (k, d, wg, nw, runs[-1])
+            if best is None or k < best[0]:
+                best = cand
+        return best
+
+    def line(app, workload, best, alg_bytes, exact, extra, cpu_s, cpu_kind):
+        k, d, wg, nw, (res, st) = best
+        ach = alg_bytes / (k * 1e6)
+        rec = {"app": app, "workload": workload, "kernel_ms": round(k, 4), "device_ms": round(d, 4),
+               "launch": {"wg_size": wg, "num_wgs": nw, "participating": st["participating"]},
+               "iterations": st["iterations"], "algorithmic_bytes": int(alg_bytes),
+               "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 5)},
+               "bitexact": exact, "cpu_oracle_ms": round(cpu_s * 1e3, 2), "cpu_oracle": cpu_kind}
+        rec.update(extra)
+        out.append(rec)
+        print(json.dumps(rec), flush=True)
+
+    # BFS on ER
+    g = G.er_config()
+    dg = rt.graph(g)
+    t0 = time.perf_counter()
+    ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.bfs(0, **lk))
+    res = best[4][0]
+    reached = ref != oracle.INF
+    ms = int(g.degrees()[reached].sum())
+    line("bfs", "ER n=2^16 avg degree 8 (m=524288), source 0", best, 8 * ms + 20 * int(reached.sum()),
+         bool(np.array_equal(res, ref)), {"edges_scanned": ms, "gteps": round(ms / (best[0] * 1e6), 4),
+                                         "us_per_level": round(best[1] * 1e3 / max(1, best[4][1]["iterations"]), 2)},
+         cpu, f"threaded frontier BFS, {threads} threads")
+    dg.close()
+    # SSSP on RMAT 20
+    g = G.rmat_config()
+    dg = rt.graph(g)
+    t0 = time.perf_counter()
+    ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.sssp(0, **lk))
+    res = best[4][0]
+    reached = ref != oracle.INF
+    ms = int(g.degrees()[reached].sum())
+    line("sssp", "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", best,
+         12 * ms + 20 * int(reached.sum()), bool(np.array_equal(res, ref)),
+         {"edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
+          "relaxations_scanned": best[4][1]["edges_scanned"]},
+         cpu, f"threaded frontier Bellman-Ford, {threads} threads")
+    dg.close()
+    # color / MIS / PageRank on the grid
+    g = G.grid_config()
+    dg = rt.graph(g)
+    n, m = g.n, g.m
+    deg = g.degrees().astype(np.int64)
+    t0 = time.perf_counter()
+    cref, crounds = oracle.color(g.row_ptr, g.col, g.prio)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.color(**lk))
+    # per round: read every color + write max_d (8n); per uncolored vertex: row pair, neighbour
+    # colors and priorities (8 + 8 deg), its priority and max (8)
+    alg = crounds * 8 * n + int((cref.astype(np.int64) * (16 + 8 * deg)).sum())
+    line("color", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], cref)),
+         {"rounds_oracle": crounds}, cpu, "serial restatement")
+    t0 = time.perf_counter()
+    sref, mrounds = oracle.mis(g.row_ptr, g.col, g.prio)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.mis(**lk))
+    # lower bound: every round reads the state of every vertex; the first reads every adjacency list
+    alg = mrounds * 4 * n + 8 * m + 4 * (n + 1)
+    line("mis", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], sref)),
+         {"rounds_oracle": mrounds}, cpu, "serial restatement")
+    t0 = time.perf_counter()
+    p64 = oracle.pagerank_f64(g.row_ptr, g.col, 0.85, 50)
+    cpu = time.perf_counter() - t0
+    p32 = oracle.pagerank_f32(g.row_ptr, g.col, np.float32(0.85), 50)
+    best = timed(lambda **lk: dg.pagerank(0.85, 50, **lk))
+    pr = best[4][0]
+    alg = 50 * (4 * (n + 1) + 8 * m + 20 * n)
+    line("pagerank", "grid 2048x2048, d=0.85, 50 iterations, FP32", best, alg, bool(np.array_equal(pr, p32)),
+         {"l1_vs_fp64": float(np.abs(pr.astype(np.float64) - p64).sum()), "bitexact_vs": "FP32 oracle"},
+         cpu, "serial FP64")
+    dg.close()
+    # barrier round trip at full occupancy
+    for var, name in ((0, "flags"), (2, "xcd"), (1, "sense")):
+        for wg in (1024, 256):
+            r = rt.barrier_bench(wg, 0, 200, var)
+            rec = {"app": "barrier", "variant": name, "wg_size": wg, "participating": r["participating"],
+                   "ns_per_barrier": round(r["ns_per_barrier"], 1), "epoch_errors": r["epoch_errors"]}
+            print(json.dumps(rec), flush=True)
+    rt.close()
+    return out
+
+
 def main():
     args = parse()
+    if args.suite == "apps":
+        suite_apps()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)

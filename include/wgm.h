@@ -57,7 +57,10 @@ typedef struct wgm_batch_item {
   uint64_t weight_seed;
 } wgm_batch_item;
 
-/* splitmix64 (Vigna): state += 0x9e3779b97f4a7c15, then the 64-bit mixer. */
+/* Synthetic inputs. No counterpart in the reference, which reads Galois .gr
+ * / DIMACS / METIS files (src/lonestar/graph_apps/bfs/bfs.h:95-169,
+ * src/pannotia/graph_parser/parse.cpp:58-281).
+ * splitmix64 (Vigna): state += 0x9e3779b97f4a7c15, then the 64-bit mixer. */
 uint64_t wgm_splitmix64_next(uint64_t* state);
 /* G(n, M): the first `pairs` distinct non-loop pairs, symmetrised. */
 int wgm_gen_er(int32_t n, int64_t pairs, uint64_t seed, wgm_csr* out);
@@ -107,6 +110,10 @@ typedef struct wgm_device_info {
 #define WGM_K_SERVE 9
 #define WGM_K_COUNT 10
 
+/* Runtime on one GPU: stream, control block (discovery poll, barrier flag
+ * lines, XCD lines) and coherent host cells. Replaces the CL_Execution set-up
+ * of src/host_skeleton/skel.cpp:633-700 and the barrier / discovery buffers
+ * and their resets (src/host_skeleton/skel.cpp:127-155, 278-295). */
 int wgm_create(int32_t device, wgm_runtime** out);
 int wgm_destroy(wgm_runtime* rt);
 const char* wgm_last_error(const wgm_runtime* rt);
@@ -114,20 +121,24 @@ const char* wgm_strerror(int32_t status);
 int wgm_get_device_info(wgm_runtime* rt, wgm_device_info* out);
 
 /* hipOccupancyMaxActiveBlocksPerMultiprocessor for `kernel` at wg_size,
- * and the CU count. Cross-check for discovery (north-star a5). */
+ * and the CU count: the vendor bound that discovery is checked against
+ * (the reference derates its discovered count instead, amd_check,
+ * src/host_skeleton/skel.cpp:242-247). */
 int wgm_occupancy_query_hip(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
                             int32_t* blocks_per_cu, int32_t* num_cus);
 
 /* Occupancy discovery probe: launch `probe_groups` workgroups of `kernel`'s
  * resource shape with im_exit set; *count = workgroups that joined the poll.
  * probe_groups <= 0 picks 2x the HIP occupancy bound.
- * Replaces get_occupancy_d_ctx (src/host_skeleton/skel.cpp:223-240). */
+ * Replaces get_occupancy_d_ctx and discovery_get_occupancy
+ * (src/host_skeleton/skel.cpp:157-165, 223-240). */
 int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
                            int32_t probe_groups, int32_t* count);
 
-/* Barrier round trip at a given occupancy (os_freq,
- * src/test_suite/os_freq/device/os_freq.cl:8-46): `iterations` x 10
- * global barriers in one launch. */
+/* Barrier round trip at a given occupancy: `iterations` x 10 global
+ * barriers in one launch. Replaces the os_freq test (kernel
+ * src/test_suite/os_freq/device/os_freq.cl:8-46, host
+ * src/test_suite/os_freq/host/os_freq.cpp:262-329). */
 #define WGM_BARRIER_FLAGS 0   /* flat master/slave flag barrier, one line per WG */
 #define WGM_BARRIER_SENSE 1   /* counter + sense reversal */
 #define WGM_BARRIER_XCD 2     /* two-level: members -> XCD leader, leaders <-> leaders */
@@ -143,12 +154,17 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
                       int32_t iterations, wgm_barrier_result* out);
 
 /* Plain device copy of `bytes` (read + write), `iterations` times: achieved
- * GB/s counts both directions. Calibrates HBM traffic counters. */
+ * GB/s counts both directions. Calibrates HBM traffic counters (no
+ * counterpart in the reference). */
 int wgm_copy_bench(wgm_runtime* rt, int64_t bytes, int32_t iterations, double* ms_per_copy, double* gbps);
 
 /* ------------------------------------------------ device-resident graphs */
 typedef struct wgm_graph wgm_graph;
-/* Upload a CSR (and optional uint32 priorities) to HBM. */
+/* Upload a CSR (and optional uint32 priorities) to HBM. Replaces the graph
+ * part of init_persistent_app_for_real / reset_persistent_task
+ * (src/lonestar/graph_apps/bfs/bfs.h:209-313,
+ * src/pannotia/graph_apps/color/color.h:41-150); the graph stays resident
+ * instead of being re-uploaded per run. */
 int wgm_graph_create(wgm_runtime* rt, const wgm_csr* g, const uint32_t* prio, wgm_graph** out);
 int wgm_graph_destroy(wgm_graph* g);
 int64_t wgm_graph_bytes(const wgm_graph* g);
@@ -170,8 +186,18 @@ typedef struct wgm_run_stats {
   double host_ms;          /* host handshake flip -> persistent_flag==0 (handshake only) */
 } wgm_run_stats;
 
-/* Standalone persistent apps. Outputs are host arrays of n entries (may be
- * NULL to leave the result on the device). */
+/* Standalone persistent apps: run_persistent (src/host_skeleton/skel.cpp:250-385)
+ * with one plugin each. Outputs are host arrays of n entries (may be NULL to
+ * leave the result on the device).
+ * wgm_bfs: Lonestar BFS (src/lonestar/graph_apps/bfs/bfs.h:16-370,
+ *   src/lonestar/graph_apps/bfs/device/standalone.cl:218-320);
+ * wgm_sssp: Lonestar SSSP (src/lonestar/graph_apps/sssp/sssp.h:16-382,
+ *   src/lonestar/graph_apps/sssp/device/standalone.cl);
+ * wgm_color: Pannotia color (src/pannotia/graph_apps/color/color.h:41-195,
+ *   src/pannotia/graph_apps/color/device/standalone.cl:10-126);
+ * wgm_mis: Pannotia MIS (src/pannotia/graph_apps/mis/mis.h:40-202,
+ *   src/pannotia/graph_apps/mis/device/standalone.cl:12-146);
+ * wgm_pagerank: no counterpart in the reference (pull PageRank, FP32). */
 int wgm_bfs(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc,
             uint32_t* dist_out, wgm_run_stats* st);
 int wgm_sssp(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc,
@@ -221,26 +247,44 @@ typedef struct wgm_mega_stats {
   double avg_persistent_groups;  /* mean of the 1 ms samples of executing persistent workgroups */
 } wgm_mega_stats;
 
-/* Prepare a mega-kernel context and the non-persistent task's inputs. */
+/* Prepare a mega-kernel context and the non-persistent tasks' inputs.
+ * Replaces mk_init_scheduler_ctx / set_scheduler_args
+ * (src/scheduler_rt/rt_host/cl_scheduler.h:7-87) and init_non_persistent_app
+ * (src/non_persistent_kernels/matmult/matmult.h:94-164,
+ * src/non_persistent_kernels/reduce/reduce.h:18-36). */
 int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out);
 int wgm_mega_destroy(wgm_mega* m);
 /* Launch the resident mega-kernel (num_wgs <= 0: discovered occupancy) and
- * wait until the scheduler is ready; *participating excludes the scheduler. */
+ * wait until the scheduler is ready; *participating excludes the scheduler.
+ * Replaces CL_Communicator::launch_mega_kernel / number_of_discovered_groups
+ * (src/scheduler_rt/rt_host/cl_communicator.h:71-99). */
 int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query_barrier, int32_t* participating);
-/* Start a persistent BFS / SSSP on `groups` workgroups (returns once assigned). */
+/* Start a persistent BFS / SSSP on `groups` workgroups (returns once
+ * assigned; a monitor thread samples the executing workgroups every 1 ms).
+ * Replaces send_persistent_task / monitor_persistent_task
+ * (src/scheduler_rt/rt_host/cl_communicator.h:168-224). */
 int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups);
+/* is_executing_persistent (src/scheduler_rt/rt_host/cl_communicator.h:226-228). */
 int wgm_mega_persistent_running(wgm_mega* m);
+/* Bounded wait for the persistent task; get_persistent_time
+ * (src/scheduler_rt/rt_host/cl_communicator.h:230-239). */
 int wgm_mega_wait_persistent(wgm_mega* m, double timeout_s, double* persistent_ms);
 /* One non-persistent task on `groups` workgroups, synchronously; checked.
- * The kind is the one given at wgm_mega_create, or `kind` (WGM_NP_*). */
+ * The kind is the one given at wgm_mega_create, or `kind` (WGM_NP_*).
+ * Replaces send_task_synchronous (src/scheduler_rt/rt_host/cl_communicator.h:123-166)
+ * followed by check_non_persistent_task (src/non_persistent_kernels/matmult/matmult.h:190-194). */
 int wgm_mega_send_task(wgm_mega* m, int32_t groups, wgm_task_times* out);
 int wgm_mega_send_task_kind(wgm_mega* m, int32_t kind, int32_t groups, wgm_task_times* out);
-/* DEVICE_TO_QUIT and wait for the kernel to finish. */
+/* DEVICE_TO_QUIT and a bounded wait for the kernel to finish: send_quit_signal
+ * (src/scheduler_rt/rt_host/cl_communicator.h:101-105) + the final queue finish. */
 int wgm_mega_quit(wgm_mega* m);
+/* The summary of print_summary (src/scheduler_rt/rt_host/cl_communicator.h:296-348),
+ * plus p50 / p99. */
 int wgm_mega_get_stats(wgm_mega* m, wgm_mega_stats* out);
 /* Result of the last persistent task (after wgm_mega_quit). */
 int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
-/* The non-persistent task launched on its own, `iterations` times. */
+/* The non-persistent task launched on its own, `iterations` times:
+ * run_non_persistent (src/host_skeleton/skel.cpp:168-221). */
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
                     int32_t* errors);
 
