@@ -145,7 +145,12 @@ class NpConfig(ctypes.Structure):
 
 class TaskTimes(ctypes.Structure):
     _fields_ = [("response_us", ctypes.c_double), ("execution_us", ctypes.c_double), ("ok", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("dev_assign_us", ctypes.c_double), ("dev_start_us", ctypes.c_double),
+                ("dev_body_us", ctypes.c_double), ("dev_retire_us", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: (bool(getattr(self, f)) if f == "ok" else getattr(self, f)) for f, _ in self._fields_
+                if f != "reserved"}
 
 
 class MegaStats(ctypes.Structure):

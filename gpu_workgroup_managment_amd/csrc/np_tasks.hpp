@@ -23,6 +23,7 @@ struct NPDev {
   const int* buf;
   int* dmin;
   int* counter;                    // finished-workgroup counter (reset per task)
+  double* partial;                 // FP32 matmul: one weighted-checksum partial per workgroup
   int* h_result;                   // coherent host cell
   unsigned long long* h_checksum;  // coherent host cell
 };
@@ -33,13 +34,13 @@ __device__ __forceinline__ int pow2_ceil(int x) {
   return p;
 }
 
-// LDS of one 256-thread slice of the MFMA matmul: an A chunk (64 rows x 16
-// k, row pitch 17) and a B chunk (16 k x 64 columns, row pitch 65), floats.
-constexpr int kMfmaSliceFloats = 64 * 17 + 16 * 65;
+// LDS of one 256-thread slice of the MFMA matmul: the accumulators of three
+// of its four waves (64 lanes x 16 floats each) for the split-K reduction.
+constexpr int kMfmaSliceFloats = 3 * 64 * 16;
 
 // Bytes of dynamic LDS the tasks need at a workgroup size: the larger of the
-// matmul slices (int32 tiles: 2 x 32 x 33 words; FP32 MFMA chunks) and the
-// FP64 checksum reduction (bd doubles).
+// matmul slices (int32 tiles: 2 x 32 x 33 words; FP32 split-K accumulators)
+// and the FP64 checksum reduction (bd doubles).
 __host__ __device__ constexpr int np_lds_bytes(int bd) {
   return (bd >= 256 ? bd / 256 : 1) * kMfmaSliceFloats * 4 > bd * 8 ? (bd >= 256 ? bd / 256 : 1) * kMfmaSliceFloats * 4
                                                                    : bd * 8;
@@ -52,62 +53,79 @@ __host__ __device__ constexpr int np_weight_v(int j) { return j % 5 + 1; }
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// FP32 square matmul on the matrix cores. A 256-thread slice computes a 64x64
-// block of C as 2x2 wave tiles of 32x32 with v_mfma_f32_32x32x2f32, streaming
-// K in chunks of 16 through LDS with one 16-byte load of A and of B per thread. Operand layout of the
+// FP32 square matmul on the matrix cores. A 256-thread slice computes one
+// 32x32 tile of C: its four waves each run v_mfma_f32_32x32x2f32 over a
+// quarter of K, with operands loaded straight from L2 in batches of 8 steps
+// (the next batch in flight while the current one computes), and the four
+// partial accumulators are summed through LDS. At dim 256 the 64 tiles keep
+// 64 slices busy, one latency round per batch. Operand layout of the
 // instruction (wave64): lane l supplies A[row l%32][k l/32] and
 // B[k l/32][col l%32]; accumulator register r of lane l holds
-// C[8(r/4) + 4(l/32) + r%4][l%32]. dim % 64 == 0.
-__device__ void np_matmul_f32_mfma(const float* A, const float* B, float* C, int dim, int vid, int nwg, float* lds) {
+// C[8(r/4) + 4(l/32) + r%4][l%32]. dim % 64 == 0. Returns this thread's share
+// of the weighted checksum.
+__device__ double np_matmul_f32_mfma(const float* A, const float* B, float* C, int dim, int vid, int nwg, float* lds) {
+  double part = 0.0;
   const int q = blockDim.x / 256;
   const int slice = threadIdx.x / 256, lt = threadIdx.x % 256;
   const int wave = lt / 64, lane = lt % 64;
-  float* As = lds + slice * kMfmaSliceFloats;
-  float* Bs = As + 64 * 17;
-  const int blocks = dim / 64;
-  const int nb = blocks * blocks;
-  const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;
-  for (int base = vid * q; base < nb; base += nwg * q) {
-    const int blk = base + slice;
-    const bool live = blk < nb;
-    const int br = live ? (blk / blocks) * 64 : 0, bc = live ? (blk % blocks) * 64 : 0;
+  float* red = lds + slice * kMfmaSliceFloats;
+  const int tiles = dim / 32;
+  const int nt = tiles * tiles;
+  const int kq = dim / 4;  // K range of one wave (a multiple of 16)
+  const int row = lane % 32, kl = lane / 32;
+  for (int base = vid * q; base < nt; base += nwg * q) {
+    const int tile = base + slice;
+    const bool live = tile < nt;
+    const int tr = live ? tile / tiles : 0, tc = live ? tile % tiles : 0;
     f32x16 acc;
     for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < dim; k0 += 16) {
-      if (live) {
-        const int ra = lt / 4, ca = (lt % 4) * 4;  // 64 rows x 4 float4
-        const float4 x = *(const float4*)&A[(br + ra) * dim + k0 + ca];
-        float* da = &As[ra * 17 + ca];
-        da[0] = x.x;
-        da[1] = x.y;
-        da[2] = x.z;
-        da[3] = x.w;
-        const int rb = lt / 16, cb = (lt % 16) * 4;  // 16 rows x 16 float4
-        const float4 y = *(const float4*)&B[(k0 + rb) * dim + bc + cb];
-        float* db = &Bs[rb * 65 + cb];
-        db[0] = y.x;
-        db[1] = y.y;
-        db[2] = y.z;
-        db[3] = y.w;
-      }
-      __syncthreads();
-      if (live) {
+    if (live) {
+      const float* ap = A + (tr * 32 + row) * dim + wave * kq + kl;  // + 2 per step
+      const float* bp = B + (wave * kq + kl) * dim + tc * 32 + row;  // + 2 dim per step
+      float a0[8], b0[8], a1[8], b1[8];
 #pragma unroll
-        for (int kk = 0; kk < 16; kk += 2) {
-          const float a = As[(wr + lane % 32) * 17 + kk + lane / 32];
-          const float b = Bs[(kk + lane / 32) * 65 + wc + lane % 32];
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      for (int i = 0; i < 8; i++) {
+        a0[i] = ap[2 * i];
+        b0[i] = bp[2 * i * dim];
+      }
+      for (int k0 = 0; k0 < kq; k0 += 16) {
+        const bool more = k0 + 16 < kq;
+        if (more) {
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            a1[i] = ap[k0 + 16 + 2 * i];
+            b1[i] = bp[(k0 + 16 + 2 * i) * dim];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[i], b0[i], acc, 0, 0, 0);
+        if (more) {
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            a0[i] = a1[i];
+            b0[i] = b1[i];
+          }
         }
       }
-      __syncthreads();
-    }
-    if (live) {
-      for (int r = 0; r < 16; r++) {
-        const int i = br + wr + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        C[i * dim + bc + wc + lane % 32] = acc[r];
+      if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) red[((wave - 1) * 16 + r) * 64 + lane] = acc[r];
       }
     }
+    __syncthreads();
+    if (live && wave == 0) {
+      const int j = tc * 32 + row;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float c = acc[r] + red[r * 64 + lane] + red[(16 + r) * 64 + lane] + red[(32 + r) * 64 + lane];
+        const int i = tr * 32 + 8 * (r / 4) + 4 * kl + (r % 4);
+        C[i * dim + j] = c;
+        part += (double)c * (double)(np_weight_u(i) * np_weight_v(j));
+      }
+    }
+    __syncthreads();
   }
+  return part;
 }
 
 // Square matmul on 32x32 output tiles staged through LDS. A workgroup of
@@ -219,15 +237,37 @@ __device__ void np_reduce_min(const NPDev& t, int vid, int nwg, int* lds) {
 // Task body + completion: the workgroup that finishes last publishes the
 // task's summary (djb2 diagonal hash / minimum / FP64 checksum) to the host
 // cell with a system-scope release.
-__device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
+// `executing` (resident kernels): the task's running-workgroup count, which
+// each workgroup decrements once its share is published; the last one after
+// it has published the task's summary.
+__device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds, unsigned long long* clk = nullptr,
+                        int* executing = nullptr) {
   const bool tiled = blockDim.x % 256 == 0;
+  if (clk && threadIdx.x == 0)
+    __hip_atomic_fetch_min(&clk[0], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (kind == WGM_NP_MATMULT_I32) {
     if (tiled) np_matmul_tiled<unsigned, int>(t.A, t.B, t.C, t.dim, vid, nwg, (unsigned*)lds);
     else np_matmul_strided<unsigned, int>(t.A, t.B, t.C, t.dim, vid, nwg);
   } else if (kind == WGM_NP_MATMUL_F32) {
-    if (tiled && t.dim % 64 == 0) np_matmul_f32_mfma(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
-    else if (tiled) np_matmul_tiled<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
-    else np_matmul_strided<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg);
+    if (tiled && t.dim % 64 == 0 && t.partial) {
+      // each workgroup reduces its share of the checksum; the last one adds
+      // the partials in workgroup order
+      double part = np_matmul_f32_mfma(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+      double* red = (double*)lds;
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = part;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < (int)blockDim.x / 64; w++) sum += red[w];
+        t.partial[vid] = sum;
+      }
+    } else if (tiled) {
+      np_matmul_tiled<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg, (float*)lds);
+    } else {
+      np_matmul_strided<float, float>(t.Af, t.Bf, t.Cf, t.dim, vid, nwg);
+    }
   } else {
     np_reduce_min(t, vid, nwg, lds);
   }
@@ -235,11 +275,17 @@ __device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
   __syncthreads();
   __shared__ int s_last;
   if (threadIdx.x == 0) {
+    if (clk)
+      __hip_atomic_fetch_max(&clk[1], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     fence_release();
     s_last = add_acq_rel(t.counter, 1) == nwg - 1;
   }
   __syncthreads();
-  if (!s_last) return;
+  if (!s_last) {
+    // this share was released by the counter increment above
+    if (executing && threadIdx.x == 0) add_relaxed(executing, -1);
+    return;
+  }
   if (kind == WGM_NP_MATMULT_I32) {
     if (threadIdx.x == 0) {
       unsigned h = 5381;
@@ -249,6 +295,15 @@ __device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
   } else if (kind == WGM_NP_REDUCE_MIN) {
     if (threadIdx.x == 0)
       __hip_atomic_store(t.h_result, ld_relaxed(t.dmin), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else if (blockDim.x % 256 == 0 && t.dim % 64 == 0 && t.partial) {
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < nwg; w++) sum += t.partial[w];
+      unsigned long long bits;
+      __builtin_memcpy(&bits, &sum, 8);
+      __hip_atomic_store(t.h_checksum, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(t.h_result, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   } else {
     // weighted checksum in FP64: sum_ij C[i][j] u(i) v(j)
     double* red = (double*)lds;
@@ -268,7 +323,8 @@ __device__ void np_task(const NPDev& t, int kind, int vid, int nwg, int* lds) {
       __hip_atomic_store(t.h_result, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  __syncthreads();
+  if (executing && threadIdx.x == 0) __hip_atomic_fetch_add(executing, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace wgm

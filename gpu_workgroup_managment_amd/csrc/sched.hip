@@ -90,12 +90,7 @@ __global__ void __launch_bounds__(1024) k_mega(Ctl c, SchedDev s, const MegaApp*
     const int task = get_task(s, did, &r);
     if (task == TASK_QUIT) break;
     if (task == TASK_MULT) {
-      np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        fence_release();
-        add_relaxed(s.np.executing, -1);
-      }
+      np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds, s.np_clock, s.np.executing);
     } else if (task == TASK_PERSIST) {
       const MegaApp app = read_app(app_cell);
       if (app.kind == WGM_PERSIST_SSSP) p_frontier<true>(s, app, r, did, lds);
@@ -133,6 +128,9 @@ __global__ void k_sched_reset(SchedDev s, int max_groups) {
     *s.p.executing = 0;
     *s.np_kind = 0;
     *s.serve_on = 0;
+    *s.np_epoch = 0;
+    *s.np_ticket = 0;
+    *s.np_k = 0;
   }
 }
 
@@ -182,9 +180,11 @@ int mega_np_setup(wgm_mega* m, const wgm_np_config* cfg) {
   t.dim = cfg->dim > 0 ? cfg->dim : 256;
   t.len = cfg->length > 0 ? cfg->length : (1 << 20);
   if (t.dim % 32 != 0) return fail(rt, WGM_ERR_INVALID, "matmul dim must be a multiple of 32");
-  HIPCHK(rt, hipMalloc(&m->np_dev, 4 * kLine * sizeof(int)));
+  // counter and minimum lines, then one FP64 checksum partial per workgroup
+  HIPCHK(rt, hipMalloc(&m->np_dev, 4 * kLine * sizeof(int) + (size_t)rt->max_grid * sizeof(double)));
   t.counter = (int*)m->np_dev;
   t.dmin = (int*)m->np_dev + kLine;
+  t.partial = (double*)((int*)m->np_dev + 4 * kLine);
   t.h_result = m->h_np_result_dev;
   t.h_checksum = (unsigned long long*)(m->h_np_checksum_dev);
   const size_t nn = (size_t)t.dim * t.dim;
@@ -321,6 +321,15 @@ int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out) {
     out->response_us = (double)(r1 - r0) * 1e-3;
     out->execution_us = (double)(r2 - r1) * 1e-3;
     out->ok = ok ? 1 : 0;
+    unsigned long long tv[5];
+    for (int i = 0; i < 5; i++)
+      tv[i] = __hip_atomic_load(&m->h_np_times[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    const double us = 1e6 / (double)rt->info.wall_clock_hz;
+    auto d = [&](int a, int b) { return tv[b] >= tv[a] ? (double)(tv[b] - tv[a]) * us : 0.0; };
+    out->dev_assign_us = d(0, 1);
+    out->dev_start_us = d(1, 2);
+    out->dev_body_us = d(2, 3);
+    out->dev_retire_us = d(3, 4);
   }
   return ok ? WGM_OK : fail(rt, WGM_ERR_MISMATCH, "non-persistent task result check failed");
 }
@@ -335,9 +344,10 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   const size_t line_b = kLine * sizeof(int);
   // host cells: flag, task_size, participating, persistent_flag, stop, np
   // result, np checksum; lines 8..15 hold the persistent task descriptor
-  hipError_t e = hipHostMalloc((void**)&m->h_cells, 16 * line_b, hipHostMallocCoherent | hipHostMallocMapped);
+  // lines 16..17: the device-clock stamps of the last task
+  hipError_t e = hipHostMalloc((void**)&m->h_cells, 24 * line_b, hipHostMallocCoherent | hipHostMallocMapped);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipHostMalloc(scheduler cells)"));
-  std::memset(m->h_cells, 0, 16 * line_b);
+  std::memset(m->h_cells, 0, 24 * line_b);
   static_assert(sizeof(MegaApp) <= 8 * kLine * sizeof(int), "task descriptor exceeds its cells");
   int* hd = nullptr;
   e = hipHostGetDevicePointer((void**)&hd, m->h_cells, 0);
@@ -361,6 +371,8 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   s.persistent_flag = hd + 3 * kLine;
   s.stop = hd + 4 * kLine;
   s.task_kind = hd + 7 * kLine;
+  m->h_np_times = (unsigned long long*)(m->h_cells + 16 * kLine);
+  s.h_np_times = (unsigned long long*)(hd + 16 * kLine);
   // device arena
   const int G = rt->max_groups;
   const size_t bytes = 16 * line_b + 2 * (size_t)G * line_b + (size_t)G * sizeof(RCtx) + 2 * (size_t)G * sizeof(int);
@@ -380,7 +392,10 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   s.p.executing = line();
   s.np_kind = line();
   s.serve_on = line();
-  p += 4 * line_b;
+  s.np_clock = (unsigned long long*)line();
+  s.np_epoch = line();
+  s.np_ticket = line();
+  s.np_k = line();
   s.task_array = (int*)p;
   p += (size_t)G * line_b;
   s.flags = (int*)p;

@@ -59,6 +59,11 @@ struct SchedDev {
   int* task_kind;       // host cell: kind of the next non-persistent task
   int* np_kind;         // device line: kind of the task being executed
   int* serve_on;        // device line: 1 while a query server is the persistent task
+  unsigned long long* np_clock;    // device line: [0] first task begin (min), [1] last share end (max)
+  int* np_epoch;        // query server: published non-persistent tasks (device line)
+  int* np_ticket;       // query server: tickets taken for the current task (device line)
+  int* np_k;            // query server: workgroups of the current task (device line)
+  unsigned long long* h_np_times;  // host cells: exec seen, assigned, begin, end, done (device clock)
   int query_barrier;    // 1: __global_barrier_resize_query semantics
   int serve;            // 1: the persistent task is a query server (any member may leave)
   int reserve;          // query server: free workgroups kept back for non-persistent tasks
@@ -235,7 +240,8 @@ __device__ __forceinline__ void assign_np(const SchedDev& s, int k, int lpg) {
     for (int i = lpg; found < k; i = (i == 1) ? lpg : i - 1) {
       if (ld_relaxed(&s.task_array[i * kLine]) != TASK_WAIT) continue;
       s.np.group_ids[i] = found++;
-      st_release(&s.task_array[i * kLine], TASK_MULT);
+      fence_release();  // group id (and the task's counters) before the mailbox
+      st_relaxed(&s.task_array[i * kLine], TASK_MULT);
       add_relaxed(s.available, -1);
     }
     return;
@@ -289,31 +295,33 @@ __device__ __forceinline__ bool await_execute(const SchedDev& s) {
 // regains workgroups only at its own barriers): while the host is idle, free
 // workgroups beyond `reserve` rejoin the server. The server has no global
 // barrier, so a member needs no restoration context or dense id.
-__device__ __forceinline__ void serve_fork_back(const SchedDev& s, int lpg) {
-  if (ld_relaxed(s.available) <= s.reserve || ld_relaxed(s.serve_on) == 0 || sys_ld_relaxed(s.stop) != 0) return;
-  if (!try_lock(s.pool_lock)) return;
-  for (int i = 1; i <= lpg && ld_relaxed(s.available) > s.reserve; i++) {
-    if (ld_relaxed(&s.task_array[i * kLine]) != TASK_WAIT) continue;
-    add_relaxed(s.p.executing, 1);
-    sys_add_release(s.persistent_flag, 1);
-    add_relaxed(s.available, -1);
-    st_release(&s.task_array[i * kLine], TASK_PERSIST);
-  }
-  sched_unlock(s.pool_lock);
+// One mailbox per call (the position is kept by the caller), so the host
+// flag is polled between any two mailbox loads.
+__device__ __forceinline__ void serve_fork_back(const SchedDev& s, int lpg, int* pos) {
+  if (ld_relaxed(s.available) <= s.reserve || ld_relaxed(s.serve_on) == 0) return;
+  const int i = *pos;
+  *pos = i >= lpg ? 1 : i + 1;
+  if (ld_relaxed(&s.task_array[i * kLine]) != TASK_WAIT) return;
+  add_relaxed(s.p.executing, 1);
+  sys_add_release(s.persistent_flag, 1);
+  add_relaxed(s.available, -1);
+  st_release(&s.task_array[i * kLine], TASK_PERSIST);
 }
 
 // src/scheduler_rt/rt_device/cl_scheduler.cl:66-171
 __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
+  int fork_pos = 1;
   while (true) {
     const int flag = host_flag(s);
-    if (s.serve && flag == DEVICE_WAITING) serve_fork_back(s, lpg);
+    if (s.serve && flag == DEVICE_WAITING) serve_fork_back(s, lpg, &fork_pos);
     if (flag == DEVICE_TO_QUIT) {
       fence_acquire();
       if (s.serve) st_relaxed(s.serve_on, 0);
-      while (ld_acquire(s.np.executing) != 0) {
+      while (ld_relaxed(s.np.executing) != 0) {
       }
-      while (ld_acquire(s.p.executing) != 0) {
+      while (ld_relaxed(s.p.executing) != 0) {
       }
+      fence_acquire();
       sched_lock(s.pool_lock);
       for (int i = 1; i < lpg + 1; i++) {
         while (ld_acquire(&s.task_array[i * kLine]) != TASK_WAIT) {
@@ -345,12 +353,34 @@ __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
       }
       // reset_non_persistent, done on the device (nothing else can run on
       // the GPU while the mega-kernel holds it)
+      const unsigned long long t_seen = wall_clock64();
       st_relaxed(s.np_kind, sys_ld_relaxed(s.task_kind));
       if (s.np_counter) st_relaxed(s.np_counter, 0);
       if (s.np_dmin) st_relaxed(s.np_dmin, 0x7fffffff);
-      assign_np(s, k, lpg);
+      if (s.np_clock) {
+        __hip_atomic_store(&s.np_clock[0], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&s.np_clock[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (s.serve) {
+        // publish: k tickets, then the epoch (one release for the whole task)
+        st_relaxed(s.np_k, k);
+        st_relaxed(s.np_ticket, 0);
+        fence_release();
+        add_relaxed(s.np_epoch, 1);
+      } else {
+        assign_np(s, k, lpg);
+      }
+      const unsigned long long t_assigned = wall_clock64();
       sched_unlock(s.pool_lock);
-      while (ld_acquire(s.np.executing) != 0) {
+      while (ld_relaxed(s.np.executing) != 0) {
+      }
+      fence_acquire();
+      if (s.h_np_times) {
+        const unsigned long long t_done = wall_clock64();
+        const unsigned long long tv[5] = {
+            t_seen, t_assigned, __hip_atomic_load(&s.np_clock[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+            __hip_atomic_load(&s.np_clock[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), t_done};
+        for (int i = 0; i < 5; i++) __hip_atomic_store(&s.h_np_times[i], tv[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }
       sys_st_release(s.scheduler_flag, DEVICE_WAITING);
     }

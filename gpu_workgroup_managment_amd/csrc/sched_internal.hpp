@@ -104,6 +104,7 @@ struct wgm_mega {
   int* h_persistent = nullptr;
   int* h_stop = nullptr;
   int* h_task_kind = nullptr;
+  unsigned long long* h_np_times = nullptr;
   int* h_np_result = nullptr;
   unsigned long long* h_np_checksum = nullptr;
   int* h_np_result_dev = nullptr;

@@ -124,12 +124,16 @@ __device__ __forceinline__ int pop_resume(const ServeDev& v) {
   return -1;
 }
 
+// Idle workgroups poll here: relaxed loads only (an acquire in a spin would
+// invalidate the XCD's L2 on every iteration); one acquire on success.
 __device__ __forceinline__ int claim_ring(const ServeDev& v) {
   int h = ld_relaxed(v.ring_head);
-  while (h < ld_acquire(v.ring_tail)) {
+  while (h < ld_relaxed(v.ring_tail)) {
     if (__hip_atomic_compare_exchange_strong(v.ring_head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return sys_ld_acquire(&v.ring[h % v.ring_cap]);
+                                             __HIP_MEMORY_SCOPE_AGENT)) {
+      fence_acquire();
+      return sys_ld_relaxed(&v.ring[h % v.ring_cap]);
+    }
   }
   return -1;
 }
@@ -311,8 +315,12 @@ __device__ void serve_loop(const SchedDev& s, const ServeDev& v, int* lds) {
 __device__ void doorbell_loop(const SchedDev& s, const ServeDev& v) {
   int last_tail = 0, last_stop = 0;
   while (ld_relaxed(v.quit_dev) == 0) {
-    const int t = sys_ld_acquire(v.ring_tail_host);
+    const int t = sys_ld_relaxed(v.ring_tail_host);
     if (t != last_tail) {
+      // the ring entries and descriptors are read by the claimer with
+      // system-scope loads after its own acquire; order the mirror after
+      // this load of the tail
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       st_release(v.ring_tail, t);
       last_tail = t;
     }
@@ -345,22 +353,42 @@ __global__ void __launch_bounds__(1024) k_serve(Ctl c, SchedDev s, ServeDev v, N
     __syncthreads();
     return;
   }
+  __shared__ int s_task[2];
+  int seen_epoch = 0;  // thread 0: last non-persistent task epoch looked at
   while (true) {
     __syncthreads();
     if (threadIdx.x == 0) {
       st_relaxed(&s.task_array[did * kLine], TASK_WAIT);
       add_relaxed(s.available, 1);
+      // Wait for the mailbox (server / quit) or a ticket of a published
+      // non-persistent task: the scheduler publishes a task with one store
+      // and the free workgroups take its k tickets in parallel.
+      int t, vid = -1;
+      while (true) {
+        t = ld_relaxed(&s.task_array[did * kLine]);
+        if (t != TASK_WAIT) break;
+        const int e = ld_relaxed(s.np_epoch);
+        if (e != seen_epoch) {
+          seen_epoch = e;
+          const int tk = add_relaxed(s.np_ticket, 1);
+          if (tk < ld_relaxed(s.np_k)) {
+            add_relaxed(s.available, -1);
+            t = TASK_MULT;
+            vid = tk;
+            break;
+          }
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      fence_acquire();
+      s_task[0] = t;
+      s_task[1] = vid;
     }
-    RCtx r;
-    const int task = get_task(s, did, &r);
+    __syncthreads();
+    const int task = s_task[0], np_vid = s_task[1];
     if (task == TASK_QUIT) break;
     if (task == TASK_MULT) {
-      np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        fence_release();
-        add_relaxed(s.np.executing, -1);
-      }
+      np_task(np, ld_relaxed(s.np_kind), np_vid, ld_relaxed(s.np_k), lds, s.np_clock, s.np.executing);
     } else if (task == TASK_PERSIST) {
       serve_loop(s, v, lds);
       drain_vmem();

@@ -72,7 +72,7 @@ class Mega:
         rc = L.lib().wgm_mega_send_task(self._h, groups, ctypes.byref(t))
         if rc not in (L.WGM_OK, -6):
             self.rt._check(rc)
-        return {"response_us": t.response_us, "execution_us": t.execution_us, "ok": bool(t.ok)}
+        return t.as_dict()
 
     def quit(self):
         self.rt._check(L.lib().wgm_mega_quit(self._h))

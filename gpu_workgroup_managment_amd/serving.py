@@ -101,7 +101,7 @@ class Server:
         rc = L.lib().wgm_serve_task(self._h, kind, groups, ctypes.byref(t))
         if rc not in (L.WGM_OK, -6):
             self.rt._check(rc)
-        return {"response_us": t.response_us, "execution_us": t.execution_us, "ok": bool(t.ok)}
+        return t.as_dict()
 
     def stop(self):
         self.rt._check(L.lib().wgm_serve_stop(self._h))

@@ -236,6 +236,11 @@ typedef struct wgm_task_times {
   double execution_us;  /* DEVICE_TO_EXECUTE -> DEVICE_WAITING */
   int32_t ok;           /* result self-check */
   int32_t reserved;
+  /* device-clock breakdown of the execution (resident kernels only) */
+  double dev_assign_us;  /* scheduler saw TO_EXECUTE -> every workgroup assigned */
+  double dev_start_us;   /* assigned -> first workgroup began the task */
+  double dev_body_us;    /* first begin -> last workgroup finished its share */
+  double dev_retire_us;  /* last share done -> scheduler saw the task complete */
 } wgm_task_times;
 
 typedef struct wgm_mega_stats {

@@ -86,6 +86,10 @@ def test_serve_mix_with_tasks_preempts_and_matches(rt, pool):
         srv.stop()
     print("tasks", tasks[:3], "preempted", sum(r["preempted"] for r in recs))
     assert all(t["ok"] for t in tasks)
+    # the device-clock breakdown covers the execution seen by the host
+    for t in tasks:
+        parts = t["dev_assign_us"] + t["dev_start_us"] + t["dev_body_us"] + t["dev_retire_us"]
+        assert t["dev_body_us"] > 0 and parts <= t["execution_us"] + 1.0, t
     _check(recs, want)
     assert len(recs) == 3 * (len(gs) + 4)
     assert sum(r["preempted"] for r in recs) > 0  # queries really yielded and resumed
