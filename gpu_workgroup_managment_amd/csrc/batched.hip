@@ -55,21 +55,24 @@ struct BatchDev {
   int max_team;    // largest team in the job table
 };
 
-// Counter barrier of a team of k workgroups (k small). Arrival is an acq_rel
-// fetch_add on a monotonic counter; the arrival that completes epoch e
-// publishes generation e; everyone then acquires.
+// Counter barrier of a team of k workgroups (k small). Arrival is a fetch_add
+// on a monotonic counter; the arrival that completes epoch e publishes
+// generation e. Everything the team shares is accessed with agent-scope
+// atomics (frontier_batch<..., COH>), so the barrier only has to drain this
+// workgroup's outstanding accesses before arriving: no L2 write-back or
+// invalidation, which would also evict every other job's cached graph on the
+// XCD.
 __device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epoch) {
   drain_vmem();
   __syncthreads();
   epoch++;
   if (threadIdx.x == 0) {
-    const int old = add_acq_rel(cnt, 1);
+    const int old = add_relaxed(cnt, 1);
     if (old == epoch * k - 1) {
-      st_release(gen, epoch);
+      st_relaxed(gen, epoch);
     } else {
       while (ld_relaxed(gen) < epoch) {
       }
-      fence_acquire();
     }
   }
   __syncthreads();
@@ -113,12 +116,13 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int r
   int epoch = 0;
   int lo, hi;
   block_range(g.n, rank, k, &lo, &hi);
+  // initial state through the coherent path (other members' atomics see it)
   for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-    s.dist[v] = (v == src) ? 0u : kInf;
-    if (SSSP) s.stamp[v] = 0u;
+    __hip_atomic_store(&s.dist[v], (v == src) ? 0u : kInf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (SSSP) __hip_atomic_store(&s.stamp[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (rank == 0 && threadIdx.x == 0) {
-    s.q0[0] = src;
+    __hip_atomic_store(&s.q0[0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st_relaxed(&s.counts[0], 1);
     st_relaxed(&s.counts[kLine], 0);
     st_relaxed(&s.counts[2 * kLine], 0);
@@ -127,7 +131,7 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int r
   team_barrier(tcnt, tgen, k, epoch);
   int level = 0;
   while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-    frontier_level<SSSP>(g, s, level, rank, k, lds);
+    frontier_level<SSSP, true>(g, s, level, rank, k, lds);
     team_barrier(tcnt, tgen, k, epoch);
     level++;
   }
@@ -436,10 +440,12 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     b->col_off.push_back(b->total_edges);
     b->v_off.push_back(b->total_v);
     b->q_off.push_back(b->total_q);
+    // per-job state on its own 256-byte lines: with team members reading
+    // through their XCD's L2, no line may be shared with another job
     b->total_rows += g.n + 1;
     b->total_edges += g.m;
-    b->total_v += g.n;
-    b->total_q += 2 * ((int64_t)g.n + 1);
+    b->total_v += ((int64_t)g.n + 63) / 64 * 64;
+    b->total_q += ((2 * ((int64_t)g.n + 1)) + 63) / 64 * 64;
   }
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 4); };
   hipError_t e = alloc((void**)&b->row, sizeof(int32_t) * b->total_rows);
@@ -457,6 +463,12 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_reached, sizeof(unsigned long long) * count);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(batch pools)"));
+  // zeroed state: a queue slot read before it is written holds a vertex id of
+  // its own job (0 at first), never garbage
+  e = hipMemset(b->dist, 0, sizeof(uint32_t) * b->total_v);
+  if (e == hipSuccess) e = hipMemset(b->stamp, 0, sizeof(uint32_t) * b->total_v);
+  if (e == hipSuccess) e = hipMemset(b->q, 0, sizeof(int32_t) * b->total_q);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMemset(batch state)"));
   // one staging copy per pool
   std::vector<int32_t> stage;
   stage.reserve((size_t)std::max(b->total_rows, b->total_edges));

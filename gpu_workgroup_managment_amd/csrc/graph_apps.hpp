@@ -55,7 +55,13 @@ __host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 
 // LOCAL: the push counter is in LDS and only this workgroup touches the
 // graph (workgroup-scope atomics). Reference: processnode2 / processedge2 /
 // wl_push_1item (src/lonestar/graph_apps/bfs/device/bfs.cl:82-212).
-template <bool SSSP, bool LOCAL>
+// COH (teams without cache maintenance at their barriers): the state other
+// workgroups write -- queue entries, a frontier vertex's own distance -- is
+// read and written with agent-scope atomic accesses, which stay coherent
+// across the XCDs' L2s without flushing or invalidating them; the read-only
+// CSR stays cached. Plain reads of dist[u] only filter candidates (a stale
+// value is never below the current one and the atomic decides).
+template <bool SSSP, bool LOCAL, bool COH = false>
 __device__ __forceinline__ void frontier_batch(const GraphDev& g, const FrontierDev& s, const int32_t* in_q, int a,
                                                int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds) {
   constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
@@ -68,10 +74,10 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   int beg = 0, deg = 0;
   uint32_t du = 0;
   if (i < b) {
-    const int v = in_q[i];
+    const int v = COH ? __hip_atomic_load(&in_q[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_q[i];
     beg = g.row_ptr[v];
     deg = g.row_ptr[v + 1] - beg;
-    if (SSSP) du = s.dist[v];
+    if (SSSP) du = COH ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : s.dist[v];
   }
   int total;
   const int excl = wg_exclusive_scan(deg, s_wave, &total);
@@ -112,7 +118,11 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
       int base = 0;
       if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
       base = __shfl(base, leader, 64);
-      if (want) out_q[base + __popcll(mask & ((1ull << lane) - 1ull))] = u;
+      if (want) {
+        int32_t* slot = &out_q[base + __popcll(mask & ((1ull << lane) - 1ull))];
+        if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *slot = u;
+      }
     }
   }
   __syncthreads();
@@ -122,7 +132,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
 // workgroups. Static mode: chunks of bd entries round-robin over the team.
 // Dynamic mode (s.chunk > 0): workgroups claim s.chunk entries at a time from
 // a per-level counter, which balances uneven chunks.
-template <bool SSSP>
+template <bool SSSP, bool COH = false>
 __device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
                                                int rank, int nteam, int* lds) {
   const int bd = blockDim.x, tid = threadIdx.x;
@@ -141,12 +151,13 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
       __syncthreads();
       if (c0 >= in_n) break;
       const int c1 = min(c0 + s.chunk, in_n);
-      for (int a = c0; a < c1; a += bd) frontier_batch<SSSP, false>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds);
+      for (int a = c0; a < c1; a += bd)
+        frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds);
     }
     if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
   } else {
     for (int a = rank * bd; a < in_n; a += nteam * bd)
-      frontier_batch<SSSP, false>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds);
+      frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds);
   }
   if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
 }
