@@ -249,6 +249,13 @@ SIGNATURES = {
     "wgm_mega_get_stats": (_i32, [_vp, _P(MegaStats)]),
     "wgm_mega_result": (_i32, [_vp, _vp, _P(_i32)]),
     "wgm_np_run_solo": (_i32, [_vp, _i32, _i32, _i32, _P(ctypes.c_double), _P(_i32)]),
+    "wgm_args_create": (_i32, [_P(_vp)]),
+    "wgm_args_push": (_i32, [_vp, _vp, _i32]),
+    "wgm_args_destroy": (_i32, [_vp]),
+    "wgm_launch_standalone": (_i32, [_vp, _vp, _i32, _i32, _i32, _vp, _i32, _P(RunStats)]),
+    "wgm_mega_launch_custom": (_i32, [_vp, _vp, _i32, _i32, _i32, _vp, _P(_i32)]),
+    "wgm_mega_send_persistent_custom": (_i32, [_vp, _i32]),
+    "wgm_mega_send_task_custom": (_i32, [_vp, _i32, _P(TaskTimes)]),
     "wgm_serve_create": (_i32, [_vp, _i32, _P(Csr), _P(ServeConfig), _P(_vp)]),
     "wgm_serve_destroy": (_i32, [_vp]),
     "wgm_serve_graph_bytes": (_i64, [_vp]),
@@ -305,3 +312,29 @@ def ptr(a):
 
 def i32ptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+class Args:
+    """wgm_args: kernel arguments in order, each a ctypes scalar or pointer
+    (copied into the builder when pushed)."""
+
+    def __init__(self, values=()):
+        self._h = ctypes.c_void_p()
+        check(lib().wgm_args_create(ctypes.byref(self._h)))
+        for v in values:
+            self.push(v)
+
+    def push(self, value):
+        check(lib().wgm_args_push(self._h, ctypes.cast(ctypes.pointer(value), ctypes.c_void_p), ctypes.sizeof(value)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().wgm_args_destroy(self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:
+            pass

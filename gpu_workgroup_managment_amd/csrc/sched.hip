@@ -86,13 +86,13 @@ __global__ void __launch_bounds__(1024) k_mega(Ctl c, SchedDev s, const MegaApp*
       st_relaxed(&s.task_array[did * kLine], TASK_WAIT);
       add_relaxed(s.available, 1);
     }
-    RCtx r;
-    const int task = get_task(s, did, &r);
+    const int task = get_task(s, did);
     if (task == TASK_QUIT) break;
     if (task == TASK_MULT) {
       np_task(np, ld_relaxed(s.np_kind), s.np.group_ids[did], k_num_groups(s.np), lds, s.np_clock, s.np.executing);
     } else if (task == TASK_PERSIST) {
       const MegaApp app = read_app(app_cell);
+      const RCtx r = *rslot<RCtx>(s.r_ctx, did);
       if (app.kind == WGM_PERSIST_SSSP) p_frontier<true>(s, app, r, did, lds);
       else p_frontier<false>(s, app, r, did, lds);
       drain_vmem();
@@ -111,7 +111,7 @@ __global__ void k_sched_reset(SchedDev s, int max_groups) {
   for (int i = t; i < max_groups; i += gridDim.x * blockDim.x) {
     s.task_array[i * kLine] = TASK_UNINIT;
     s.flags[i * kLine] = 0;
-    s.r_ctx[i].target = 0;
+    *(int*)rslot<RCtx>(s.r_ctx, i) = 0;
     s.np.group_ids[i] = -1;
     s.p.group_ids[i] = -1;
   }
@@ -302,8 +302,11 @@ int mega_enqueue_sched_reset(wgm_mega* m) {
 // server: reset, TO_TASK, GOT_GROUPS, TO_EXECUTE, WAITING, check.
 int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out) {
   wgm_runtime* rt = m->rt;
-  int rc = mega_np_reset(m, kind, false);
-  if (rc) return rc;
+  const bool builtin = kind >= 0;  // kind < 0: a caller's task, checked by the caller
+  if (builtin) {
+    int rc = mega_np_reset(m, kind, false);
+    if (rc) return rc;
+  }
   m->np_groups = groups;
   hstore(m->h_task_size, groups);
   const uint64_t r0 = now_ns();
@@ -313,7 +316,7 @@ int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out) {
   hstore(m->h_flag, DEVICE_TO_EXECUTE);
   if (!hwait(m->h_flag, DEVICE_WAITING, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_WAITING");
   const uint64_t r2 = now_ns();
-  const bool ok = mega_np_check(m, kind);
+  const bool ok = builtin ? mega_np_check(m, kind) : true;
   if (!ok) m->errors++;
   m->response_us.push_back((double)(r1 - r0) * 1e-3);
   m->exec_us.push_back((double)(r2 - r1) * 1e-3);
@@ -375,7 +378,7 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   s.h_np_times = (unsigned long long*)(hd + 16 * kLine);
   // device arena
   const int G = rt->max_groups;
-  const size_t bytes = 16 * line_b + 2 * (size_t)G * line_b + (size_t)G * sizeof(RCtx) + 2 * (size_t)G * sizeof(int);
+  const size_t bytes = 16 * line_b + 2 * (size_t)G * line_b + (size_t)G * kRSlotBytes + 2 * (size_t)G * sizeof(int);
   e = hipMalloc(&m->d_arena, bytes);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(scheduler arena)"));
   char* p = (char*)m->d_arena;
@@ -400,8 +403,8 @@ int wgm_mega_create(wgm_runtime* rt, const wgm_np_config* np, wgm_mega** out) {
   p += (size_t)G * line_b;
   s.flags = (int*)p;
   p += (size_t)G * line_b;
-  s.r_ctx = (RCtx*)p;
-  p += (size_t)G * sizeof(RCtx);
+  s.r_ctx = (unsigned char*)p;
+  p += (size_t)G * kRSlotBytes;
   s.np.group_ids = (int*)p;
   p += (size_t)G * sizeof(int);
   s.p.group_ids = (int*)p;
@@ -491,7 +494,16 @@ int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t sr
   std::memcpy((void*)m->h_app, &app, sizeof(app));
   std::atomic_thread_fence(std::memory_order_release);
   m->graph = g;
-  // monitor_persistent_task (src/scheduler_rt/rt_host/cl_communicator.h:168-205)
+  return mega_start_persistent(m, groups);
+}
+
+}  // extern "C"
+
+// send_persistent_task + monitor_persistent_task
+// (src/scheduler_rt/rt_host/cl_communicator.h:168-224): assign the persistent
+// task's workgroups, then a monitor thread samples the executing ones.
+int mega_start_persistent(wgm_mega* m, int groups) {
+  wgm_runtime* rt = m->rt;
   hstore(m->h_task_size, groups);
   hstore(m->h_flag, DEVICE_TO_PERSISTENT_TASK);
   if (!hwait(m->h_flag, DEVICE_GOT_GROUPS, rt->timeout_s)) return fail(rt, WGM_ERR_TIMEOUT, "no DEVICE_GOT_GROUPS");
@@ -512,6 +524,8 @@ int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t sr
   });
   return WGM_OK;
 }
+
+extern "C" {
 
 int wgm_mega_persistent_running(wgm_mega* m) { return (m && m->persist_running.load()) ? 1 : 0; }
 

@@ -27,6 +27,16 @@ struct RCtx {
 };
 static_assert(sizeof(RCtx) == 8, "restoration context is one 8-byte word");
 
+// Every workgroup owns one restoration slot of kRSlotBytes in device memory.
+// A persistent task stores there whatever context type it uses (its first
+// member is the int resume target); the runtime's own tasks use RCtx.
+constexpr int kRSlotBytes = 64;
+template <typename R>
+__device__ __forceinline__ R* rslot(unsigned char* slots, int did) {
+  static_assert(sizeof(R) <= kRSlotBytes, "restoration context exceeds its slot");
+  return (R*)(slots + (size_t)did * kRSlotBytes);
+}
+
 // Per-task virtual workgroup ids (Kernel_ctx,
 // src/scheduler_rt/rt_common/kernel_ctx.h:6-10).
 struct KCtx {
@@ -51,7 +61,7 @@ struct SchedDev {
   int* flags;           // resizable barrier flags, by persistent virtual id (lines)
   int* bar_groups;      // bar->num_groups: group count of the current phase
   int* bar_to_kill;     // query variant: kills decided by the master
-  RCtx* r_ctx;          // [max_groups]
+  unsigned char* r_ctx;  // [max_groups] restoration slots of kRSlotBytes
   KCtx np;
   KCtx p;
   int* np_counter;      // non-persistent task: finished-workgroup counter (reset per task)
@@ -82,19 +92,18 @@ __device__ __forceinline__ int k_num_groups(const KCtx& k) { return ld_relaxed(k
 // ------------------------------------------------------------- get_task
 // Thread 0 spins on this workgroup's mailbox; the restoration context is
 // copied through LDS to every thread (src/scheduler_rt/rt_device/cl_scheduler.cl:31-50).
-__device__ __forceinline__ int get_task(const SchedDev& s, int did, RCtx* r_out) {
+// The restoration context is left in the workgroup's slot (rslot); a task
+// reads it from there after the acquire above.
+__device__ __forceinline__ int get_task(const SchedDev& s, int did) {
   __shared__ int s_task;
-  __shared__ RCtx s_r;
   if (threadIdx.x == 0) {
     int t;
     while ((t = ld_relaxed(&s.task_array[did * kLine])) == TASK_WAIT) {
     }
     fence_acquire();
     s_task = t;
-    s_r = s.r_ctx[did];
   }
   __syncthreads();
-  *r_out = s_r;
   const int t = s_task;
   __syncthreads();
   return t;
@@ -135,7 +144,8 @@ __device__ __forceinline__ bool offer_kill(const SchedDev& s, int vid) {
 // context and TASK_PERSIST, then grow num_groups / executing /
 // persistent_flag (src/scheduler_rt/rt_device/scheduler_1.cl:10-57).
 // Returns the group count before the fork; *after = count after.
-__device__ __forceinline__ int offer_fork_thread0(const SchedDev& s, RCtx to_fork, int* after) {
+template <typename R>
+__device__ __forceinline__ int offer_fork_thread0(const SchedDev& s, const R& to_fork, int* after) {
   const int pre = k_num_groups(s.p);
   *after = pre;
   if (ld_relaxed(s.available) > 0 && try_lock(s.pool_lock)) {
@@ -145,7 +155,7 @@ __device__ __forceinline__ int offer_fork_thread0(const SchedDev& s, RCtx to_for
       s.p.group_ids[i + 1] = i;
       while (ld_relaxed(&s.task_array[(i + 1) * kLine]) != TASK_WAIT) {
       }
-      s.r_ctx[i + 1] = to_fork;
+      *rslot<R>(s.r_ctx, i + 1) = to_fork;
       st_release(&s.task_array[(i + 1) * kLine], TASK_PERSIST);
     }
     add_relaxed(s.p.num_groups, snapshot);
@@ -167,7 +177,8 @@ __device__ __forceinline__ int offer_fork_thread0(const SchedDev& s, RCtx to_for
 // to leave. Returns true if the caller must leave the persistent task.
 // Query variant: the master reads groups_to_kill and shrinks the team
 // itself; condemned slaves (highest ids) leave after the release.
-__device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, RCtx to_fork) {
+template <typename R>
+__device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, const R& to_fork) {
   __shared__ int s_b[2];
   drain_vmem();
   __syncthreads();
@@ -262,7 +273,7 @@ __device__ __forceinline__ void assign_p(const SchedDev& s, int k) {
     s.p.group_ids[i + 1] = i;
     while (ld_relaxed(&s.task_array[(i + 1) * kLine]) != TASK_WAIT) {
     }
-    s.r_ctx[i + 1].target = 0;
+    *(int*)rslot<RCtx>(s.r_ctx, i + 1) = 0;  // target 0: start from the top
     st_release(&s.task_array[(i + 1) * kLine], TASK_PERSIST);
     add_relaxed(s.available, -1);
   }

@@ -132,3 +132,4 @@ struct wgm_mega {
 // shared with the query server (serve.hip)
 int mega_run_task(wgm_mega* m, int kind, int groups, wgm_task_times* out);
 int mega_enqueue_sched_reset(wgm_mega* m);
+int mega_start_persistent(wgm_mega* m, int groups);

@@ -86,6 +86,15 @@ class Runtime:
     def graph(self, csr):
         return DeviceGraph(self, csr)
 
+    def launch_standalone(self, kernel, args=None, wg_size=256, num_wgs=0, lds_bytes=0, handshake=False):
+        """Run the caller's standalone persistent kernel (its host handle as an
+        int / c_void_p; first parameter wgm_kernel_args) with `args` (L.Args)."""
+        st = L.RunStats()
+        self._check(L.lib().wgm_launch_standalone(self.handle, ctypes.c_void_p(kernel), wg_size, num_wgs, lds_bytes,
+                                                  args.handle if args is not None else None, 1 if handshake else 0,
+                                                  ctypes.byref(st)))
+        return st.as_dict()
+
 
 def _launch(wg_size=256, num_wgs=0, handshake=False):
     lc = L.Launch()

@@ -88,6 +88,23 @@ class Mega:
         self.rt._check(L.lib().wgm_mega_result(self._h, L.ptr(dist), ctypes.byref(lv)))
         return dist, lv.value
 
+    def launch_custom(self, kernel, args=None, wg_size=256, num_wgs=0, lds_bytes=0):
+        """Launch the caller's merged mega-kernel (first parameter wgm_mega_args)."""
+        p = ctypes.c_int32()
+        self.rt._check(L.lib().wgm_mega_launch_custom(self._h, ctypes.c_void_p(kernel), wg_size, num_wgs, lds_bytes,
+                                                      args.handle if args is not None else None, ctypes.byref(p)))
+        self.participating = p.value
+        return p.value
+
+    def send_persistent_custom(self, groups=None):
+        groups = self.participating if groups is None else groups
+        self.rt._check(L.lib().wgm_mega_send_persistent_custom(self._h, groups))
+
+    def send_task_custom(self, groups):
+        t = L.TaskTimes()
+        self.rt._check(L.lib().wgm_mega_send_task_custom(self._h, groups, ctypes.byref(t)))
+        return t.as_dict()
+
     def run_solo(self, wg_size=256, num_wgs=8, iterations=10):
         us, errs = ctypes.c_double(), ctypes.c_int32()
         self.rt._check(L.lib().wgm_np_run_solo(self._h, wg_size, num_wgs, iterations, ctypes.byref(us), ctypes.byref(errs)))

@@ -293,6 +293,37 @@ int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
                     int32_t* errors);
 
+/* --------------------------------------------- the caller's own kernels
+ * Persistent kernels written against include/wgm_device_api.h (the
+ * reference's scheduler API: kernel_merge/stubs/stubs.h:4-7,
+ * src/scheduler_rt/rt_device/iw_barrier.cl, scheduler_1.cl, kernel_ctx.cl) and
+ * compiled by the caller for gfx950.
+ * `kernel` is the kernel's host-side handle (its __global__ function). */
+typedef struct wgm_args wgm_args;
+/* Ordered kernel arguments (after the runtime's own first argument), in
+ * place of cl::Kernel::setArg with a running index
+ * (src/lonestar/graph_apps/bfs/bfs.h:265-283). Each push copies `bytes`. */
+int wgm_args_create(wgm_args** out);
+int wgm_args_push(wgm_args* a, const void* value, int32_t bytes);
+int wgm_args_destroy(wgm_args* a);
+/* Standalone persistent kernel: first parameter wgm_kernel_args, then the
+ * pushed arguments. num_wgs <= 0: the workgroups a discovery probe of the
+ * kernel finds co-resident. handshake: the host flag handshake of
+ * run_persistent (src/host_skeleton/skel.cpp:300-345). */
+int wgm_launch_standalone(wgm_runtime* rt, const void* kernel, int32_t wg_size, int32_t num_wgs, int32_t lds_bytes,
+                          const wgm_args* args, int32_t handshake, wgm_run_stats* st);
+/* A caller's merged mega-kernel (first parameter wgm_mega_args, body
+ * WGM_MEGA_BODY): launch_mega_kernel (cl_communicator.h:71-95) on the
+ * scheduler state of `m`. */
+int wgm_mega_launch_custom(wgm_mega* m, const void* kernel, int32_t wg_size, int32_t num_wgs, int32_t lds_bytes,
+                           const wgm_args* args, int32_t* participating);
+/* send_persistent_task for the kernel's own persistent task
+ * (cl_communicator.h:207-224). */
+int wgm_mega_send_persistent_custom(wgm_mega* m, int32_t groups);
+/* send_task_synchronous for the kernel's own non-persistent task
+ * (cl_communicator.h:123-166); the caller checks the task's output. */
+int wgm_mega_send_task_custom(wgm_mega* m, int32_t groups, wgm_task_times* out);
+
 /* --------------------------------------------------------- query server
  * Serving (no counterpart in the reference, whose driver sends one
  * non-persistent task at a time beside one persistent task,

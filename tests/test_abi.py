@@ -12,8 +12,9 @@ import pytest
 from gpu_workgroup_managment_amd import _lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the C-ABI headers (wgm_device_api.h is HIP device code for the caller's kernels)
 HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
-           if f.endswith(".h")]
+           if f.endswith(".h") and "device" not in f]
 
 STRUCTS = {
     "wgm_csr": L.Csr,
@@ -116,3 +117,20 @@ def test_dist_digest_host_function():
     assert got == want
     # order independence is what lets the device sum it in any order
     assert L.lib().wgm_dist_digest(None, 0) == 0
+
+
+def test_device_api_header_compiles_for_gfx950(tmp_path):
+    """The public device API builds a caller's kernel (host compile + gfx950)."""
+    src = tmp_path / "k.hip"
+    src.write_text('''#include "wgm_device_api.h"
+__global__ void k(wgm_kernel_args wa, int* out) {
+  DISCOVERY_PROTOCOL(wa);
+  if (get_local_id(0) == 0) out[get_group_id(0)] = get_num_groups(0);
+  global_barrier();
+  WGM_STANDALONE_END(wa);
+}
+''')
+    cmd = ["/opt/rocm/bin/hipcc", "-std=c++17", "--offload-arch=gfx950", "-c", str(src), "-o", str(tmp_path / "k.o"),
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "gpu_workgroup_managment_amd", "csrc")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
