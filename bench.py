@@ -22,9 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Tuned on MI355X by wgm_batch_autotune over this workload (wg 64..1024,
-# 1..8 workgroups per CU, team granularity, chunk 32..4096).
-TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 16, max_team=16)
+# Tuned on MI355X over this workload (wg 64..1024, 1..8 workgroups per CU,
+# team granularity 2^14..2^22 edges, chunk 32..4096, SSSP near-far band
+# 0..16384): 1024 x 2 per CU, teams of ceil(m / 2^20), band 1024.
+TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=1024)
 
 
 def parse():
@@ -308,7 +309,7 @@ def main():
     cfg_d = dict(TUNED)
     if args.autotune:
         best, _ = batch.autotune(1)
-        cfg_d = {k: v for k, v in best.as_dict().items() if k != "use_graph"}
+        cfg_d = {k: v for k, v in best.as_dict().items() if k not in ("use_graph", "reserved")}
     cfg = B.config(use_graph=True, **cfg_d)
 
     for _ in range(args.warmup):

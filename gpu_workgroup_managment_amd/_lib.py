@@ -117,6 +117,8 @@ class BatchConfig(ctypes.Structure):
         ("team_edges", ctypes.c_int32),
         ("max_team", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("delta", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):

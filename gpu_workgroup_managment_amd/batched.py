@@ -9,10 +9,11 @@ from . import _lib as L
 from . import graphs as G
 
 
-def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False):
+def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False, delta=0):
     c = L.BatchConfig()
     c.wg_size, c.wgs_per_cu, c.chunk = wg_size, wgs_per_cu, chunk
     c.team_edges, c.max_team, c.use_graph = team_edges, max_team, 1 if use_graph else 0
+    c.delta = delta
     return c
 
 

@@ -31,6 +31,7 @@ struct BatchJob {
   long long col_off;  // into the col / wt pools
   long long v_off;    // into the per-vertex pools (dist, stamp)
   long long q_off;    // into the queue pool (2 * (n + 1) entries)
+  long long f_off;    // into the far-queue pool (2 * (n + 1) entries) and the in_far pool (n)
   int n, m, query, team, src, ticket;
   int index;          // caller's graph index
   int pad;
@@ -53,7 +54,37 @@ struct BatchDev {
   int* ticket;     // one line
   int* iters;      // per job
   int max_team;    // largest team in the job table
+  int32_t* far;    // near-far: far queues
+  uint32_t* in_far;
+  int* fcnt;       // 3 lines per job: far counts and band minimum
+  uint32_t delta;  // near-far band width (0: off)
 };
+
+// Wave-aggregated push of `item` where `want`, one counter atomic per wave.
+template <bool COH>
+__device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* cnt, int scope) {
+  const unsigned long long mask = __ballot(want);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) {
+    base = scope == __HIP_MEMORY_SCOPE_WORKGROUP
+               ? __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+               : __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  base = __shfl(base, leader, 64);
+  if (want) {
+    int32_t* slot = &q[base + __popcll(mask & ((1ull << lane) - 1ull))];
+    if (COH) __hip_atomic_store(slot, item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *slot = item;
+  }
+}
+
+__device__ __forceinline__ int wave_min_int(int x) {
+  for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
+  return x;
+}
 
 // Counter barrier of a team of k workgroups (k small). Arrival is a fetch_add
 // on a monotonic counter; the arrival that completes epoch e publishes
@@ -80,39 +111,88 @@ __device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epo
 
 // A graph served by one workgroup: level counters live in LDS and the
 // workgroup's own barrier orders the levels; all atomics are workgroup scope.
+// SSSP with nf.delta > 0 runs near-far bands: levels relax only below the
+// band's threshold; when the near frontier empties, the threshold moves to
+// the smallest deferred distance + delta and the far queue is split.
 template <bool SSSP>
-__device__ void run_single(const GraphDev& g, const FrontierDev& s, int src, int* lds, int* s_cnt, int* iters_out) {
+__device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
+                           int* iters_out) {
+  const bool near_far = SSSP && nf.delta > 0;
+  __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
     if (SSSP) s.stamp[v] = 0u;
+    if (near_far) nf.flag[v] = 0u;
   }
   if (threadIdx.x == 0) {
     s.q0[0] = src;
     s_cnt[0] = 1;
     s_cnt[1] = 0;
     s_cnt[2] = 0;
+    s_far[0] = 0;
+    s_far[1] = 0;
+    s_far[2] = INT32_MAX;
   }
   __syncthreads();
   const int bd = blockDim.x;
-  int level = 0;
-  for (;; level++) {
-    const int in_n = s_cnt[level % 3];
-    if (in_n == 0) break;
-    const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
-    int32_t* out_q = (level & 1) ? s.q0 : s.q1;
-    for (int a = 0; a < in_n; a += bd)
-      frontier_batch<SSSP, true>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3], (uint32_t)level + 1u,
-                                 lds);
-    if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
+  int level = 0, fi = 0;
+  uint32_t thr = near_far ? nf.delta : 0xffffffffu;
+  while (true) {
+    for (;; level++) {
+      const int in_n = s_cnt[level % 3];
+      if (in_n == 0) break;
+      const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
+      int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+      for (int a = 0; a < in_n; a += bd)
+        frontier_batch<SSSP, true>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
+                                   (uint32_t)level + 1u, lds, thr, near_far ? nf.q[fi] : nullptr, &s_far[fi], nf.flag);
+      if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
+      __syncthreads();
+    }
+    if (!near_far) break;
+    const int fn = s_far[fi];
+    if (fn == 0) break;
+    int mn = INT32_MAX;
+    for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.q[fi][i]]);
+    mn = wave_min_int(mn);
+    if ((threadIdx.x & 63) == 0) atomicMin(&s_far[2], mn);
+    __syncthreads();
+    thr = (uint32_t)s_far[2] + nf.delta;
+    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    for (int base = 0; base < fn; base += bd) {
+      const int i = base + threadIdx.x;
+      bool to_near = false, to_far = false;
+      int v = 0;
+      if (i < fn) {
+        v = nf.q[fi][i];
+        if (s.dist[v] < thr) {
+          nf.flag[v] = 0u;
+          to_near = __hip_atomic_fetch_max(&s.stamp[v], (uint32_t)level, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) < (uint32_t)level;
+        } else {
+          to_far = true;  // stays flagged in the other far queue
+        }
+      }
+      push_item<false>(to_near, v, lq, &s_cnt[level % 3], __HIP_MEMORY_SCOPE_WORKGROUP);
+      push_item<false>(to_far, v, nf.q[1 - fi], &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_far[fi] = 0;
+      s_far[2] = INT32_MAX;
+    }
+    fi = 1 - fi;
     __syncthreads();
   }
   if (threadIdx.x == 0) *iters_out = level;
 }
 
-// A graph served by a team of k > 1 workgroups.
+// A graph served by a team of k > 1 workgroups (near-far as in run_single,
+// with the far counters and band minimum in device lines).
 template <bool SSSP>
-__device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int rank, int k, int* tcnt, int* tgen,
-                         int* lds, int* iters_out) {
+__device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int rank, int k,
+                         int* tcnt, int* tgen, int* lds, int* iters_out) {
+  const bool near_far = SSSP && nf.delta > 0;
   int epoch = 0;
   int lo, hi;
   block_range(g.n, rank, k, &lo, &hi);
@@ -120,6 +200,7 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int r
   for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
     __hip_atomic_store(&s.dist[v], (v == src) ? 0u : kInf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (SSSP) __hip_atomic_store(&s.stamp[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (near_far) __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (rank == 0 && threadIdx.x == 0) {
     __hip_atomic_store(&s.q0[0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -127,13 +208,62 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, int src, int r
     st_relaxed(&s.counts[kLine], 0);
     st_relaxed(&s.counts[2 * kLine], 0);
     for (int l = 0; l < 3; l++) st_relaxed(&s.claims[l * kLine], 0);
+    if (near_far) {
+      st_relaxed(&nf.cnt[0], 0);
+      st_relaxed(&nf.cnt[kLine], 0);
+      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
+    }
   }
   team_barrier(tcnt, tgen, k, epoch);
-  int level = 0;
-  while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-    frontier_level<SSSP, true>(g, s, level, rank, k, lds);
+  const int bd = blockDim.x;
+  int level = 0, fi = 0;
+  uint32_t thr = near_far ? nf.delta : 0xffffffffu;
+  while (true) {
+    while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
+      frontier_level<SSSP, true>(g, s, level, rank, k, lds, thr, near_far ? nf.q[fi] : nullptr, &nf.cnt[fi * kLine],
+                                 nf.flag);
+      team_barrier(tcnt, tgen, k, epoch);
+      level++;
+    }
+    if (!near_far) break;
+    const int fn = ld_relaxed(&nf.cnt[fi * kLine]);
+    if (fn == 0) break;
+    int mn = INT32_MAX;
+    for (int i = rank * bd + (int)threadIdx.x; i < fn; i += k * bd) {
+      const int v = __hip_atomic_load(&nf.q[fi][i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mn = min(mn, (int)__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    mn = wave_min_int(mn);
+    if ((threadIdx.x & 63) == 0 && mn != INT32_MAX)
+      __hip_atomic_fetch_min(&nf.cnt[2 * kLine], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     team_barrier(tcnt, tgen, k, epoch);
-    level++;
+    thr = (uint32_t)ld_relaxed(&nf.cnt[2 * kLine]) + nf.delta;
+    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    for (int base = rank * bd; base < fn; base += k * bd) {
+      const int i = base + (int)threadIdx.x;
+      bool to_near = false, to_far = false;
+      int v = 0;
+      if (i < fn) {
+        v = __hip_atomic_load(&nf.q[fi][i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < thr) {
+          __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          to_near = __hip_atomic_fetch_max(&s.stamp[v], (uint32_t)level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                    (uint32_t)level;
+        } else {
+          to_far = true;
+        }
+      }
+      push_item<true>(to_near, v, lq, &s.counts[(level % 3) * kLine], __HIP_MEMORY_SCOPE_AGENT);
+      push_item<true>(to_far, v, nf.q[1 - fi], &nf.cnt[(1 - fi) * kLine], __HIP_MEMORY_SCOPE_AGENT);
+    }
+    team_barrier(tcnt, tgen, k, epoch);
+    // every member has read q[fi] and the minimum; the next split follows at
+    // least one more team barrier (the minimum moved to the near queue)
+    if (rank == 0 && threadIdx.x == 0) {
+      st_relaxed(&nf.cnt[fi * kLine], 0);
+      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
+    }
+    fi = 1 - fi;
   }
   if (rank == 0 && threadIdx.x == 0) *iters_out = level;
 }
@@ -190,14 +320,22 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     s.claims = b.claims + (size_t)j * 3 * kLine;
     s.chunk = b.chunk;
     const bool sssp = jb.query == WGM_QUERY_SSSP;
+    NearFar nf{};
+    nf.delta = sssp && b.far ? b.delta : 0u;
+    if (nf.delta) {
+      nf.q[0] = b.far + jb.f_off;
+      nf.q[1] = b.far + jb.f_off + jb.n + 1;
+      nf.flag = b.in_far + jb.v_off;
+      nf.cnt = b.fcnt + (size_t)j * 3 * kLine;
+    }
     if (jb.team <= 1) {
-      if (sssp) run_single<true>(g, s, jb.src, lds, s_cnt, &b.iters[j]);
-      else run_single<false>(g, s, jb.src, lds, s_cnt, &b.iters[j]);
+      if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+      else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
     } else {
       int* tcnt = b.tbar + (size_t)j * 2 * kLine;
       int* tgen = tcnt + kLine;
-      if (sssp) run_team<true>(g, s, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
-      else run_team<false>(g, s, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
+      if (sssp) run_team<true>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
+      else run_team<false>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
     }
     __syncthreads();
   }
@@ -246,6 +384,12 @@ struct wgm_batch {
   int chunk = 0;
   int32_t *ticket = nullptr, *iters = nullptr;
   uint32_t *dist = nullptr, *stamp = nullptr;
+  int32_t* far = nullptr;         // near-far queues, 2 (n + 1) per job (aligned)
+  uint32_t* in_far = nullptr;     // near-far flags, per vertex (v_off)
+  int32_t* fcnt = nullptr;        // 3 lines per job
+  int64_t total_f = 0;
+  std::vector<int64_t> f_off;
+  uint32_t delta = 0;             // near-far band of the current job table
   BatchJob* d_jobs = nullptr;
   unsigned long long* d_reached = nullptr;
   std::vector<BatchJob> jobs;           // current job table (LPT order)
@@ -296,6 +440,10 @@ BatchDev dev_view(wgm_batch* b) {
   d.tbar = b->tbar;
   d.ticket = b->ticket;
   d.iters = b->iters;
+  d.far = b->far;
+  d.in_far = b->in_far;
+  d.fcnt = b->fcnt;
+  d.delta = b->delta;
   return d;
 }
 
@@ -307,6 +455,8 @@ int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out
   if (c.max_team <= 0) c.max_team = 16;
   if (c.team_edges <= 0) c.team_edges = 1 << 16;
   if (c.chunk <= 0) c.chunk = c.wg_size;
+  if (c.delta < 0) c.delta = 0;
+  c.reserved = 0;
   int rc = wgm_check_wg(rt, c.wg_size);
   if (rc) return rc;
   if (c.wgs_per_cu > 32 || c.max_team > 1024) return fail(rt, WGM_ERR_INVALID, "bad batch config");
@@ -328,6 +478,7 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.col_off = b->col_off[i];
     j.v_off = b->v_off[i];
     j.q_off = b->q_off[i];
+    j.f_off = b->f_off[i];
     j.n = b->n[i];
     j.m = (int)b->m[i];
     j.query = b->query[i];
@@ -338,6 +489,7 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     ticket += j.team;
   }
   b->chunk = cfg.chunk;
+  b->delta = (uint32_t)cfg.delta;
   b->jobs_cfg = cfg;
   b->jobs_grid = grid;
   WGM_HIPCHK(b->rt, hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * b->count, hipMemcpyHostToDevice));
@@ -440,12 +592,14 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     b->col_off.push_back(b->total_edges);
     b->v_off.push_back(b->total_v);
     b->q_off.push_back(b->total_q);
+    b->f_off.push_back(b->total_f);
     // per-job state on its own 256-byte lines: with team members reading
     // through their XCD's L2, no line may be shared with another job
     b->total_rows += g.n + 1;
     b->total_edges += g.m;
     b->total_v += ((int64_t)g.n + 63) / 64 * 64;
     b->total_q += ((2 * ((int64_t)g.n + 1)) + 63) / 64 * 64;
+    b->total_f += ((2 * ((int64_t)g.n + 1)) + 63) / 64 * 64;
   }
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 4); };
   hipError_t e = alloc((void**)&b->row, sizeof(int32_t) * b->total_rows);
@@ -462,12 +616,17 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->iters, sizeof(int32_t) * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_reached, sizeof(unsigned long long) * count);
+  if (e == hipSuccess) e = alloc((void**)&b->far, sizeof(int32_t) * b->total_f);
+  if (e == hipSuccess) e = alloc((void**)&b->in_far, sizeof(uint32_t) * b->total_v);
+  if (e == hipSuccess) e = alloc((void**)&b->fcnt, sizeof(int32_t) * 3 * kLine * count);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(batch pools)"));
   // zeroed state: a queue slot read before it is written holds a vertex id of
   // its own job (0 at first), never garbage
   e = hipMemset(b->dist, 0, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = hipMemset(b->stamp, 0, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = hipMemset(b->q, 0, sizeof(int32_t) * b->total_q);
+  if (e == hipSuccess) e = hipMemset(b->far, 0, sizeof(int32_t) * b->total_f);
+  if (e == hipSuccess) e = hipMemset(b->in_far, 0, sizeof(uint32_t) * b->total_v);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMemset(batch state)"));
   // one staging copy per pool
   std::vector<int32_t> stage;
@@ -495,7 +654,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->stamp,
-                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one};
+                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -635,7 +794,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
   if (reps < 1) reps = 1;
   const int wgs[] = {64, 128, 256, 512, 1024};
   const int per_cu[] = {1, 2, 4, 8};
-  const int team_edges[] = {1 << 14, 1 << 16, 1 << 18};
+  const int team_edges[] = {1 << 14, 1 << 16, 1 << 18, 1 << 20};
   const int chunks[] = {32, 256, 4096};
   wgm_batch_config best{};
   wgm_batch_stats bst{};
@@ -653,6 +812,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
           c.chunk = ch;
           c.max_team = 16;
           c.use_graph = 0;
+          c.delta = 1024;
           for (int r = 0; r < reps; r++) {
             wgm_batch_stats s{};
             int rc = wgm_batch_run(rt, b, &c, &s);
@@ -665,6 +825,22 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
           }
         }
     }
+  // then the SSSP near-far band at the best launch shape
+  const int deltas[] = {0, 512, 2048, 4096};
+  for (int dl : deltas) {
+    wgm_batch_config c = best;
+    c.delta = dl;
+    for (int r = 0; r < reps; r++) {
+      wgm_batch_stats s{};
+      int rc = wgm_batch_run(rt, b, &c, &s);
+      if (rc) return rc;
+      if (s.wall_ms < best_ms) {
+        best_ms = s.wall_ms;
+        best = c;
+        bst = s;
+      }
+    }
+  }
   *best_out = best;
   if (best_stats) *best_stats = bst;
   return WGM_OK;

@@ -61,9 +61,16 @@ __host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 
 // across the XCDs' L2s without flushing or invalidating them; the read-only
 // CSR stays cached. Plain reads of dist[u] only filter candidates (a stale
 // value is never below the current one and the atomic decides).
+// Near-far (SSSP, far_q != nullptr): a vertex whose distance drops to
+// alt >= thr is not queued for the next level but deferred to far_q, at most
+// once while its in_far flag is set; the caller processes it in a later
+// band. Final distances are the same (they are unique); the work is closer
+// to Dijkstra's order.
 template <bool SSSP, bool LOCAL, bool COH = false>
 __device__ __forceinline__ void frontier_batch(const GraphDev& g, const FrontierDev& s, const int32_t* in_q, int a,
-                                               int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds) {
+                                               int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds,
+                                               uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
+                                               int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
   constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
   const int bd = blockDim.x, tid = threadIdx.x;
   int* s_end = lds;
@@ -87,7 +94,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   __syncthreads();
   for (int eb = 0; eb < total; eb += bd) {
     const int e = eb + tid;
-    bool want = false;
+    bool want = false, far_want = false;
     int u = 0;
     if (e < total) {
       int lo = 0, hi = bd - 1;
@@ -107,7 +114,10 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
         const uint32_t alt = s_du[lo] + (uint32_t)g.wt[idx];
         if (alt < s.dist[u]) {
           const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
-          if (alt < old) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+          if (alt < old) {
+            if (alt < thr) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+            else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
+          }
         }
       }
     }
@@ -124,9 +134,33 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
         else *slot = u;
       }
     }
+    if (SSSP && far_q) {
+      const unsigned long long fmask = __ballot(far_want);
+      if (fmask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)fmask) - 1;
+        int base = 0;
+        if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
+        base = __shfl(base, leader, 64);
+        if (far_want) {
+          int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
+          if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          else *slot = u;
+        }
+      }
+    }
   }
   __syncthreads();
 }
+
+// Near-far state of one traversal: two far queues (n + 1 entries each) and
+// their counters, the in_far flags, the band minimum and width.
+struct NearFar {
+  int32_t* q[2];
+  int* cnt;          // lines: [0] count of q[0], [kLine] count of q[1], [2 kLine] band minimum
+  uint32_t* flag;    // in_far[n]
+  uint32_t delta;    // band width; 0: plain frontier Bellman-Ford
+};
 
 // One level of frontier relaxation over q[level & 1] by a team of nteam
 // workgroups. Static mode: chunks of bd entries round-robin over the team.
@@ -134,7 +168,9 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
 // a per-level counter, which balances uneven chunks.
 template <bool SSSP, bool COH = false>
 __device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
-                                               int rank, int nteam, int* lds) {
+                                               int rank, int nteam, int* lds, uint32_t thr = 0xffffffffu,
+                                               int32_t* far_q = nullptr, int* far_cnt = nullptr,
+                                               uint32_t* in_far = nullptr) {
   const int bd = blockDim.x, tid = threadIdx.x;
   const int in_n = ld_relaxed(&s.counts[(level % 3) * kLine]);
   const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
@@ -152,12 +188,14 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
       if (c0 >= in_n) break;
       const int c1 = min(c0 + s.chunk, in_n);
       for (int a = c0; a < c1; a += bd)
-        frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds);
+        frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr, far_q,
+                                         far_cnt, in_far);
     }
     if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
   } else {
     for (int a = rank * bd; a < in_n; a += nteam * bd)
-      frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds);
+      frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr, far_q,
+                                       far_cnt, in_far);
   }
   if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
 }

@@ -25,11 +25,13 @@ def _check_all(batch, graphs, refs):
         assert edges == int(g.degrees()[refs[i] != oracle.INF].sum())
 
 
+@pytest.mark.parametrize("delta", [0, 64, 2048])
 @pytest.mark.parametrize("team_edges,chunk", [(1 << 30, 0), (512, 0), (512, 64)])
-def test_batched_persistent(rt, small_batch, team_edges, chunk):
+def test_batched_persistent(rt, small_batch, team_edges, chunk, delta):
+    # delta > 0: SSSP near-far bands (single workgroups and teams); 64 forces many band splits
     graphs, queries, refs = small_batch
     b = B.Batch(rt, graphs, queries)
-    st = b.run(wg_size=256, wgs_per_cu=2, team_edges=team_edges, chunk=chunk, max_team=8)
+    st = b.run(wg_size=256, wgs_per_cu=2, team_edges=team_edges, chunk=chunk, max_team=8, delta=delta)
     print(st)
     assert st["launches"] == 1
     _check_all(b, graphs, refs)
