@@ -325,10 +325,17 @@ def main():
     edges = st["edges_scanned"]
     gteps = edges / (ms_per_step * 1e6)
 
-    relaunch = None
-    for _ in range(args.relaunch_steps):
-        r = batch.run_relaunch(B.config(**cfg_d))
-        relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
+    # Baseline: one persistent launch per graph, each graph's launch as wide
+    # as its team. The best of three team granularities is reported (the
+    # batched shape's coarse teams would understate it).
+    relaunch, relaunch_tried = None, []
+    for shape in (dict(cfg_d), dict(cfg_d, team_edges=1 << 16, max_team=16), dict(cfg_d, team_edges=1 << 14,
+                                                                                   max_team=64)):
+        for _ in range(args.relaunch_steps):
+            r = batch.run_relaunch(B.config(**shape))
+            r["shape"] = {"team_edges": shape["team_edges"], "max_team": shape["max_team"]}
+            relaunch_tried.append({"gteps": round(r["gteps"], 4), **r["shape"]})
+            relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
     # the relaunch overwrote results: run the batched launch once more for the checks below
     batch.run(cfg)
 
@@ -383,6 +390,8 @@ def main():
             "gteps": round(relaunch["gteps"], 4),
             "ms": round(relaunch["wall_ms"], 3),
             "launches": relaunch["launches"],
+            "shape": relaunch["shape"],
+            "tried": relaunch_tried,
             "speedup_batched_vs_relaunch": round(gteps / relaunch["gteps"], 2) if relaunch["gteps"] else None,
         },
         "copy_calibration": copy_cal,

@@ -22,7 +22,10 @@ struct wgm_runtime {
   int* h_cells_dev = nullptr;  // device view of the same cells
   int max_groups = 0;          // flag lines (participants)
   int max_grid = 0;            // p_group_ids entries (launch grid)
-  double window_us = 50.0;
+  // Discovery quiescence window: only a grid larger than what is resident
+  // (a probe) waits it out, once; a short window under-counts when the
+  // dispatch brings workgroups up slowly (first use of a kernel, profiling).
+  double window_us = 1000.0;
   double timeout_s = 60.0;
   wgm::Ctl ctl{};              // device view, passed to kernels
   std::map<int, int> occ_cache;   // HIP occupancy per (kernel, wg)
