@@ -23,10 +23,20 @@ ERRORS = {
     -7: "WGM_ERR_TIMEOUT",
     -8: "WGM_ERR_STATE",
     -9: "WGM_ERR_BUDGET",
+    -10: "WGM_ERR_IO",
+    -11: "WGM_ERR_FORMAT",
 }
+WGM_ERR_MISMATCH, WGM_ERR_IO, WGM_ERR_FORMAT = -6, -10, -11
+
+# file formats (WGM_FMT_*, WGM_READ_*, WGM_SOL_*)
+FMT_GR, FMT_DIMACS, FMT_METIS, FMT_MM, FMT_EDGES = range(5)
+READ_UNDIRECTED, READ_TRANSPOSE = 1, 2
+SOL_COLON, SOL_COLON_SPACE, SOL_PLAIN = range(3)
 
 # kernel ids (WGM_K_*)
-K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED, K_SERVE = range(10)
+(K_DISCOVERY, K_BARRIER_BENCH, K_BFS, K_SSSP, K_COLOR, K_MIS, K_PAGERANK, K_MEGA, K_BATCHED, K_SERVE, K_MINPLUS,
+ K_BC) = range(12)
+BIG_NUM = 99999999  # WGM_BIG_NUM: Pannotia SSSP's unreached distance
 GRAPH_RMAT, GRAPH_GRID = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
 BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD = 0, 1, 2
@@ -219,6 +229,13 @@ SIGNATURES = {
     "wgm_gen_weights": (_i32, [_P(Csr), _i32, _i32, _u64]),
     "wgm_gen_priorities": (_i32, [_P(Csr), _u64, _vp]),
     "wgm_csr_free": (None, [_P(Csr)]),
+    "wgm_read_graph": (_i32, [ctypes.c_char_p, _i32, _i32, _P(Csr)]),
+    "wgm_write_graph": (_i32, [ctypes.c_char_p, _i32, _i32, _P(Csr)]),
+    "wgm_format_error": (ctypes.c_char_p, []),
+    "wgm_write_solution": (_i32, [ctypes.c_char_p, _i32, _vp, _i32]),
+    "wgm_check_solution": (_i32, [ctypes.c_char_p, _i32, _vp, _i32, _P(_i32)]),
+    "wgm_write_solution_f32": (_i32, [ctypes.c_char_p, _vp, _i32]),
+    "wgm_check_solution_f32": (_i32, [ctypes.c_char_p, _vp, _i32, ctypes.c_float, _P(_i32)]),
     "wgm_free": (None, [_vp]),
     "wgm_batch_spec": (_i32, [_i32, _u64, _i32, _i32, _P(BatchItem)]),
     "wgm_batch_generate": (_i32, [_i32, _P(BatchItem), _P(Csr), _i32]),
@@ -239,6 +256,8 @@ SIGNATURES = {
     "wgm_color": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_mis": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_pagerank": (_i32, [_vp, _vp, ctypes.c_float, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_sssp_minplus": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_bc": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
     "wgm_mega_create": (_i32, [_vp, _P(NpConfig), _P(_vp)]),
     "wgm_mega_destroy": (_i32, [_vp]),
     "wgm_mega_launch": (_i32, [_vp, _i32, _i32, _i32, _P(_i32)]),

@@ -226,7 +226,7 @@ bool wgm_app_kernel_info(int kernel, KernelInfo* out) {
     case WGM_K_COLOR: *out = KernelInfo{(const void*)&k_color, nullptr}; return true;
     case WGM_K_MIS: *out = KernelInfo{(const void*)&k_mis, nullptr}; return true;
     case WGM_K_PAGERANK: *out = KernelInfo{(const void*)&k_pagerank, nullptr}; return true;
-    default: return false;
+    default: return wgm_pannotia_kernel_info(kernel, out);
   }
 }
 
@@ -256,7 +256,7 @@ int wgm_app_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
                          (float*)nullptr, (float*)nullptr, 0.f, 0.f, 0.f, 0, (int*)nullptr);
       return WGM_OK;
     default:
-      return WGM_ERR_INVALID;
+      return wgm_pannotia_probe(rt, kernel, grid, wg_size);
   }
 }
 
@@ -294,6 +294,7 @@ int wgm_graph_create(wgm_runtime* rt, const wgm_csr* h, const uint32_t* prio, wg
 
 int wgm_graph_destroy(wgm_graph* g) {
   if (!g) return WGM_OK;
+  wgm_pannotia_release(g);
   void* ptrs[] = {g->row_ptr, g->col, g->wt, g->prio, g->dist, g->stamp, g->q0, g->q1, g->counts,
                   g->i32a, g->i32b, g->f0, g->f1, g->f2, g->iters};
   for (void* p : ptrs)
@@ -307,6 +308,7 @@ int64_t wgm_graph_bytes(const wgm_graph* g) {
   int64_t b = 4 * ((int64_t)g->n + 1) + 4 * g->m;
   if (g->wt) b += 4 * g->m;
   if (g->prio) b += 4 * (int64_t)g->n;
+  if (g->rrow) b += 4 * ((int64_t)g->n + 1) + (g->rwt ? 8 : 4) * g->m + 16 * (int64_t)g->ntasks;
   return b;
 }
 

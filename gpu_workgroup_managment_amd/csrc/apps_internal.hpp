@@ -23,8 +23,26 @@ struct wgm_graph {
   float* f1 = nullptr;
   float* f2 = nullptr;
   int* iters = nullptr;
+  // in-edge CSR and min-plus tasks (Pannotia SSSP / BC), built on first use
+  int32_t* rrow = nullptr;
+  int32_t* rcol = nullptr;
+  int32_t* rwt = nullptr;
+  void* tasks = nullptr;  // int4 {row, begin, end, 0}
+  int32_t ntasks = 0;
+  // BC per-slot state and the result
+  int32_t bc_slots = 0;
+  int32_t* bc_d = nullptr;
+  double* bc_sig = nullptr;
+  double* bc_del = nullptr;
+  int32_t* bc_q = nullptr;
+  int32_t* bc_lvl = nullptr;
+  double* bc_sum = nullptr;
 };
 
 wgm::GraphDev wgm_graph_view(const wgm_graph* g);
+// defined by the Pannotia unit (pannotia.hip)
+bool wgm_pannotia_kernel_info(int kernel, KernelInfo* out);
+int wgm_pannotia_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
+void wgm_pannotia_release(wgm_graph* g);
 int wgm_graph_frontier_state(wgm_graph* g, bool sssp, wgm::FrontierDev* s);
 int wgm_graph_reached_edges(wgm_graph* g, int64_t* out);

@@ -257,6 +257,8 @@ const char* wgm_strerror(int32_t status) {
     case WGM_ERR_TIMEOUT: return "timed out";
     case WGM_ERR_STATE: return "invalid protocol state";
     case WGM_ERR_BUDGET: return "HBM budget exceeded";
+    case WGM_ERR_IO: return "file I/O error";
+    case WGM_ERR_FORMAT: return "malformed file";
     default: return "unknown status";
   }
 }
@@ -352,7 +354,20 @@ int wgm_create(int32_t device, wgm_runtime** out) {
 int wgm_destroy(wgm_runtime* rt) {
   if (!rt) return WGM_OK;
   (void)hipSetDevice(rt->device);
-  if (rt->stream) (void)hipStreamSynchronize(rt->stream);
+  if (rt->stream) {
+    // Bounded like every other host wait: a kernel that never finishes
+    // (a caller's persistent kernel that deadlocked) must not hang the
+    // teardown. Its resources are then left to process exit.
+    const auto t0 = std::chrono::steady_clock::now();
+    while (hipStreamQuery(rt->stream) == hipErrorNotReady) {
+      if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > rt->timeout_s) {
+        std::fprintf(stderr, "wgm_destroy: work on the runtime stream did not finish in %.0f s; resources not freed\n",
+                     rt->timeout_s);
+        return WGM_ERR_TIMEOUT;
+      }
+      std::this_thread::yield();
+    }
+  }
   wgm_runtime_release_extras(rt);
   if (rt->ev0) (void)hipEventDestroy(rt->ev0);
   if (rt->ev1) (void)hipEventDestroy(rt->ev1);

@@ -126,8 +126,15 @@ __device__ __forceinline__ bool offer_kill(const SchedDev& s, int vid) {
       if (to_kill > 0 && __hip_atomic_compare_exchange_strong(s.groups_to_kill, &to_kill, to_kill - 1,
                                                               __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_AGENT)) {
-        __hip_atomic_fetch_sub(s.p.num_groups, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        killed = 1;
+        // Leave only while still the highest id: a fork can take the pool
+        // lock once the scheduler's gather is complete, and it grows
+        // num_groups from the value it reads. If it got in first, the kill
+        // unit is dropped (the scheduler no longer waits for it).
+        int expect = vid + 1;
+        killed = __hip_atomic_compare_exchange_strong(s.p.num_groups, &expect, vid, __ATOMIC_RELEASE,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     ? 1
+                     : 0;
       }
     }
     s_k = killed;
@@ -177,8 +184,10 @@ __device__ __forceinline__ int offer_fork_thread0(const SchedDev& s, const R& to
 // to leave. Returns true if the caller must leave the persistent task.
 // Query variant: the master reads groups_to_kill and shrinks the team
 // itself; condemned slaves (highest ids) leave after the release.
+// resizable = false: the plain barrier over the persistent task's current
+// workgroups (global_barrier in a merged kernel): no kill, no fork.
 template <typename R>
-__device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, const R& to_fork) {
+__device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, const R& to_fork, bool resizable = true) {
   __shared__ int s_b[2];
   drain_vmem();
   __syncthreads();
@@ -190,9 +199,11 @@ __device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, const
     fence_acquire();
     __syncthreads();
     if (threadIdx.x == 0) {
-      const int to_kill = s.query_barrier ? ld_relaxed(s.groups_to_kill) : 0;
+      const int to_kill = s.query_barrier && resizable ? ld_relaxed(s.groups_to_kill) : 0;
       int former, after, kill = 0;
-      if (to_kill <= 0) {
+      if (!resizable) {
+        former = after = k_num_groups(s.p);
+      } else if (to_kill <= 0) {
         former = offer_fork_thread0(s, to_fork, &after);
       } else {
         former = k_num_groups(s.p);
@@ -212,7 +223,7 @@ __device__ __forceinline__ bool resize_barrier(const SchedDev& s, int vid, const
     __syncthreads();
     return false;
   }
-  if (!s.query_barrier && offer_kill(s, vid)) return true;
+  if (resizable && !s.query_barrier && offer_kill(s, vid)) return true;
   if (threadIdx.x == 0) {
     int* f = &s.flags[vid * kLine];
     fence_release();

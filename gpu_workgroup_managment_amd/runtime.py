@@ -150,3 +150,12 @@ class DeviceGraph:
     def pagerank(self, damping=0.85, iterations=50, **lk):
         return self._run(L.lib().wgm_pagerank, ctypes.c_float(damping), iterations,
                          out=np.empty(self.csr.n, np.float32), **lk)
+
+    def sssp_minplus(self, src=0, **lk):
+        """Pannotia SSSP (min-plus sweeps over in-edges): int32 distances,
+        L.BIG_NUM where unreached."""
+        return self._run(L.lib().wgm_sssp_minplus, src, out=np.empty(self.csr.n, np.int32), **lk)
+
+    def bc(self, num_sources=0, **lk):
+        """Pannotia BC (Brandes, FP64) over sources [0, num_sources), all when 0."""
+        return self._run(L.lib().wgm_bc, num_sources, out=np.empty(self.csr.n, np.float64), **lk)

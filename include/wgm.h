@@ -30,6 +30,8 @@ extern "C" {
 #define WGM_ERR_TIMEOUT (-7)          /* a bounded host wait expired */
 #define WGM_ERR_STATE (-8)            /* call not valid in the current protocol state */
 #define WGM_ERR_BUDGET (-9)           /* admission refused: HBM budget exceeded */
+#define WGM_ERR_IO (-10)              /* a file could not be opened, read or written */
+#define WGM_ERR_FORMAT (-11)          /* a graph or solution file is malformed; see wgm_format_error */
 
 /* ------------------------------------------------------- graphs (host CSR) */
 /* 0-based CSR: row_ptr[n+1], col[m] ascending per row, optional wt[m]. */
@@ -77,6 +79,56 @@ int wgm_gen_priorities(const wgm_csr* g, uint64_t seed, uint32_t* prio);
 void wgm_csr_free(wgm_csr* g);
 /* free() for single arrays the library allocated (e.g. wgm_gen_weights' wt). */
 void wgm_free(void* p);
+/* ------------------------------------------------------------ file formats
+ * The reference's datasets and solution files, read into (and written from)
+ * the 0-based CSR above. Readers allocate out's arrays (wgm_csr_free); out->wt
+ * is NULL when the file carries no weights. Every reader builds a complete
+ * row_ptr and keeps each row's edges in file order.
+ *   WGM_FMT_GR      Galois .gr v1 binary, 4-byte or no edge data
+ *                   (readFromGR, src/lonestar/graph_apps/bfs/bfs.h:95-169)
+ *   WGM_FMT_DIMACS  "p sp n m" / "a u v w", 1-based
+ *                   (parseCOO, src/pannotia/graph_parser/parse.cpp:177-281)
+ *   WGM_FMT_METIS   "n m [fmt]" then one adjacency line per vertex, fmt 0 or 1
+ *                   (parseMetis, parse.cpp:58-175)
+ *   WGM_FMT_MM      MatrixMarket coordinate, self-loops dropped
+ *                   (parseMM, parse.cpp:467-580)
+ *   WGM_FMT_EDGES   "n m" then "src dst [w]" lines, 0-based
+ *                   (readFromEdges, src/lonestar/graph_apps/sssp/sssp.h:49-94)
+ * Flags: WGM_READ_UNDIRECTED adds the reverse of every arc (the reference's
+ * directed = false; for METIS it means each edge is listed at both ends and
+ * m counts edges once); WGM_READ_TRANSPOSE gives the transpose
+ * (parseCOO_transpose, parse.cpp:692-797). */
+#define WGM_FMT_GR 0
+#define WGM_FMT_DIMACS 1
+#define WGM_FMT_METIS 2
+#define WGM_FMT_MM 3
+#define WGM_FMT_EDGES 4
+#define WGM_READ_UNDIRECTED 1
+#define WGM_READ_TRANSPOSE 2
+int wgm_read_graph(const char* path, int32_t format, int32_t flags, wgm_csr* out);
+/* Writers for .gr, DIMACS (every CSR edge as an arc) and METIS (flags
+ * WGM_READ_UNDIRECTED: m written as half the CSR edges). */
+int wgm_write_graph(const char* path, int32_t format, int32_t flags, const wgm_csr* g);
+/* Message of the last failed format / solution call on this thread. */
+const char* wgm_format_error(void);
+
+/* Solution files, one vertex per line:
+ *   WGM_SOL_COLON        "%d:%d" index from 0 (Lonestar, bfs.h:336-365)
+ *   WGM_SOL_COLON_SPACE  "%d: %d" index from 1 (Pannotia color / SSSP,
+ *                        src/pannotia/graph_apps/color/color.h:149-190)
+ *   WGM_SOL_PLAIN        "%d" (Pannotia MIS, mis.h:170-195)
+ * wgm_check_solution is diff_solution_file_int: the first n values must
+ * match; WGM_ERR_MISMATCH (or WGM_ERR_FORMAT for a short or malformed file)
+ * with *first_bad = the vertex. The _f32 pair is BC's "%f" file compared
+ * within an absolute tolerance (bc.h:176-203 uses 8). */
+#define WGM_SOL_COLON 0
+#define WGM_SOL_COLON_SPACE 1
+#define WGM_SOL_PLAIN 2
+int wgm_write_solution(const char* path, int32_t style, const int32_t* values, int32_t n);
+int wgm_check_solution(const char* path, int32_t style, const int32_t* values, int32_t n, int32_t* first_bad);
+int wgm_write_solution_f32(const char* path, const float* values, int32_t n);
+int wgm_check_solution_f32(const char* path, const float* values, int32_t n, float tol, int32_t* first_bad);
+
 /* Batched workload description (count items) and parallel generation. */
 int wgm_batch_spec(int32_t count, uint64_t seed, int32_t lo_exp, int32_t hi_exp,
                    wgm_batch_item* items);
@@ -108,7 +160,9 @@ typedef struct wgm_device_info {
 #define WGM_K_MEGA 7
 #define WGM_K_BATCHED 8
 #define WGM_K_SERVE 9
-#define WGM_K_COUNT 10
+#define WGM_K_MINPLUS 10
+#define WGM_K_BC 11
+#define WGM_K_COUNT 12
 
 /* Runtime on one GPU: stream, control block (discovery poll, barrier flag
  * lines, XCD lines) and coherent host cells. Replaces the CL_Execution set-up
@@ -208,6 +262,27 @@ int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
             wgm_run_stats* st);
 int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations,
                  const wgm_launch* lc, float* rank_out, wgm_run_stats* st);
+
+/* Pannotia SSSP: run_persistent with the sssp plugin
+ * (src/pannotia/graph_apps/sssp/sssp.h:40-176, kernel sssp_combined,
+ * src/pannotia/graph_apps/sssp/device/sssp.cl:11-92). Min-plus sweeps over the
+ * in-edges (the reference reads its graph transposed, sssp.h:43; here the
+ * in-edge CSR is built from the graph on first use and kept) until no
+ * distance changes. dist_out: 0 at src, WGM_BIG_NUM (sssp.h:3) where
+ * unreached. st->iterations = sweeps, st->edges_scanned = sweeps * m. */
+#define WGM_BIG_NUM 99999999
+int wgm_sssp_minplus(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, int32_t* dist_out,
+                     wgm_run_stats* st);
+/* Pannotia BC: run_persistent with the bc plugin (src/pannotia/graph_apps/bc/bc.h,
+ * kernel bc_combined, src/pannotia/graph_apps/bc/device/bc.cl:186-235).
+ * Brandes' betweenness over sources [0, num_sources) (all when <= 0), in
+ * FP64 without the reference's x10000 scaling; bc_out[v] = sum over those
+ * sources s != v of the dependency of s on v (directed, unnormalised).
+ * Deterministic: equal to the same sums in source order on the CPU.
+ * st->iterations = BFS levels over all sources, st->edges_scanned =
+ * num_sources * m. */
+int wgm_bc(wgm_runtime* rt, wgm_graph* g, int32_t num_sources, const wgm_launch* lc, double* bc_out,
+           wgm_run_stats* st);
 
 /* ------------------------------------------ resizing scheduler (mega-kernel)
  * One resident mega-kernel: discovered workgroup 0 runs the scheduler, the

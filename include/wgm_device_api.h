@@ -72,7 +72,13 @@ __device__ inline bool begin_standalone(const wgm_kernel_args& wa) {
   wgm::Team t;  // per thread: the barrier advances every thread's own epoch copy
   const bool in = wgm::standalone_begin(wa.ctl, t);
   if (in && threadIdx.x == 0) {
-    standalone_state().ctl = wa.ctl;
+    // Word by word: a struct copy into LDS goes through a private temporary,
+    // which the backend then promotes to LDS once per lane of the largest
+    // workgroup (32 KB per workgroup, a cap on occupancy).
+    static_assert(sizeof(wgm::Ctl) % 8 == 0, "Ctl is copied in 8-byte words");
+    const unsigned long long* src = (const unsigned long long*)&wa.ctl;
+    unsigned long long* dst = (unsigned long long*)&standalone_state().ctl;
+    for (int i = 0; i < (int)(sizeof(wgm::Ctl) / 8); i++) dst[i] = src[i];
     standalone_state().team = t;
   }
   __syncthreads();
@@ -150,6 +156,39 @@ __device__ inline int k_get_global_id(const Kernel_ctx* k) {
 template <typename R>
 __device__ inline bool global_barrier_resize(const wgm_mega_args& ma, const Kernel_ctx* k, const R& to_fork) {
   return wgm::resize_barrier<R>(ma.sched, k_get_group_id(k), to_fork);
+}
+
+/* Plain barrier over the persistent task's current workgroups, with no kill
+ * and no fork (global_barrier of a merged kernel, the form kernel_merge
+ * gives it in ProcessPersistentKernelVisitor.cpp:96-101; the group count is
+ * re-read while waiting, so it stays correct across resizes). */
+__device__ inline void wgm_global_barrier(const wgm_mega_args& ma, const Kernel_ctx* k) {
+  wgm::resize_barrier<wgm::RCtx>(ma.sched, k_get_group_id(k), wgm::RCtx{0, 0}, false);
+}
+
+/* offer_kill (src/scheduler_rt/rt_device/scheduler_1.cl:59-81,138): true when
+ * the scheduler wants workgroups back and this one, the task's highest id,
+ * must leave the persistent task (the caller returns). */
+__device__ inline bool wgm_offer_kill(const wgm_mega_args& ma, const Kernel_ctx* k) {
+  return wgm::offer_kill(ma.sched, k_get_group_id(k));
+}
+
+/* offer_fork (scheduler_1.cl:10-57): free workgroups join the persistent
+ * task and start from `to_fork`. Only the task's workgroup 0 forks: it also
+ * leads every barrier, so a fork never runs while a barrier is counting. */
+template <typename R>
+__device__ inline void wgm_offer_fork(const wgm_mega_args& ma, const Kernel_ctx* k, const R& to_fork) {
+  const bool lead = k_get_group_id(k) == 0;
+  if (lead) {
+    wgm::drain_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      wgm::fence_release();
+      int after;
+      wgm::offer_fork_thread0<R>(ma.sched, to_fork, &after);
+    }
+    __syncthreads();
+  }
 }
 
 /* The restoration context this workgroup was started or forked with. */

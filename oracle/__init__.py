@@ -45,6 +45,9 @@ def lib():
             "or_bfs_threads": (i32, [i32, _i32p, _i32p, i32, _u32p, i32]),
             "or_sssp_threads": (i32, [i32, _i32p, _i32p, _i32p, i32, _u32p, i32]),
             "or_fnv1a64": (ctypes.c_uint64, [ctypes.c_void_p, i64]),
+            "or_reverse_csr": (i32, [i32, _i32p, _i32p, ctypes.c_void_p, _i32p, _i32p, ctypes.c_void_p]),
+            "or_sssp_minplus": (i32, [i32, _i32p, _i32p, _i32p, i32, _i32p]),
+            "or_bc": (i64, [i32, _i32p, _i32p, _i32p, _i32p, i32, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -163,3 +166,47 @@ def fnv1a64(data):
         return lib().or_fnv1a64(ctypes.cast(buf, ctypes.c_void_p), len(data))
     a = np.ascontiguousarray(data)
     return lib().or_fnv1a64(a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+
+BIG_NUM = 99999999
+
+
+def _c(a, dtype=np.int32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def reverse_csr(row_ptr, col, wt=None):
+    """In-edge CSR (rrow, rcol, rwt): sources ascending within each row."""
+    row_ptr, col = _c(row_ptr), _c(col)
+    n, m = len(row_ptr) - 1, len(col)
+    rrow = np.empty(n + 1, np.int32)
+    rcol = np.empty(max(m, 1), np.int32)
+    w = _c(wt) if wt is not None else None
+    rwt = np.empty(max(m, 1), np.int32) if wt is not None else None
+    _check(lib().or_reverse_csr(n, row_ptr, col, w.ctypes.data if w is not None else None, rrow, rcol,
+                                rwt.ctypes.data if rwt is not None else None), "or_reverse_csr")
+    return rrow, rcol[:m], (rwt[:m] if rwt is not None else None)
+
+
+def sssp_minplus(row_ptr, col, wt, src=0):
+    """Pannotia SSSP on the out-edge CSR (transposed here, as the reference
+    reads its input): (dist int32 with BIG_NUM for unreached, sweeps)."""
+    rrow, rcol, rwt = reverse_csr(row_ptr, col, wt)
+    n = len(rrow) - 1
+    out = np.empty(n, np.int32)
+    sweeps = _check(lib().or_sssp_minplus(n, rrow, rcol, rwt if rwt is not None else np.zeros(1, np.int32), src, out),
+                    "or_sssp_minplus")
+    return out, sweeps
+
+
+def bc(row_ptr, col, num_sources=None):
+    """Pannotia BC in FP64 over sources [0, num_sources): (bc, total levels)."""
+    row_ptr, col = _c(row_ptr), _c(col)
+    n = len(row_ptr) - 1
+    rrow, rcol, _ = reverse_csr(row_ptr, col)
+    out = np.empty(n, np.float64)
+    ns = n if num_sources is None else num_sources
+    lv = lib().or_bc(n, row_ptr, col, rrow, rcol if len(rcol) else np.zeros(1, np.int32), ns, out)
+    if lv < 0:
+        raise RuntimeError(f"or_bc failed ({lv})")
+    return out, int(lv)

@@ -480,6 +480,136 @@ int or_sssp_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, const
 /* ---------------------------------------------------------------- digest
  * FNV-1a, 64-bit: offset basis 0xcbf29ce484222325, prime 0x100000001b3
  * (Fowler/Noll/Vo, published parameters). */
+/* ------------------------------------------------------ reverse CSR
+ * Counting sort by destination, sources visited in ascending order: the
+ * in-edge lists that parseCOO_transpose builds
+ * (src/pannotia/graph_parser/parse.cpp:692-797, stable sort by row). */
+int or_reverse_csr(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* wt, int32_t* rrow,
+                   int32_t* rcol, int32_t* rwt) {
+  if (n <= 0) return -1;
+  const int32_t m = row_ptr[n];
+  memset(rrow, 0, sizeof(int32_t) * ((size_t)n + 1));
+  for (int32_t e = 0; e < m; e++) rrow[col[e] + 1]++;
+  for (int32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
+  int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  if (!pos) return -2;
+  memcpy(pos, rrow, sizeof(int32_t) * (size_t)n);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t e = row_ptr[u]; e < row_ptr[u + 1]; e++) {
+      const int32_t p = pos[col[e]]++;
+      rcol[p] = u;
+      if (rwt && wt) rwt[p] = wt[e];
+    }
+  free(pos);
+  return 0;
+}
+
+/* ---------------------------------------------------- Pannotia SSSP
+ * sssp_combined (src/pannotia/graph_apps/sssp/device/sssp.cl:11-92): per
+ * iteration x = y; y[v] = min(x[v], min_j data[j] + x[col[j]]) over the
+ * in-edge row of v (the graph is read transposed, sssp.h:43); stop when y
+ * equals x. Start values: 0 at the source, BIG_NUM elsewhere (sssp.h:3,
+ * 61-68). */
+int or_sssp_minplus(int32_t n, const int32_t* rrow, const int32_t* rcol, const int32_t* rwt, int32_t src,
+                    int32_t* dist) {
+  if (n <= 0 || src < 0 || src >= n) return -1;
+  int32_t* x = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  if (!x) return -2;
+  for (int32_t v = 0; v < n; v++) dist[v] = v == src ? 0 : OR_BIG_NUM;
+  int sweeps = 0;
+  for (;;) {
+    memcpy(x, dist, sizeof(int32_t) * (size_t)n);
+    int changed = 0;
+    for (int32_t v = 0; v < n; v++) {
+      int32_t mn = x[v];
+      for (int32_t j = rrow[v]; j < rrow[v + 1]; j++) {
+        const int32_t c = rwt[j] + x[rcol[j]];
+        if (c < mn) mn = c;
+      }
+      dist[v] = mn;
+      changed |= mn != x[v];
+    }
+    sweeps++;
+    if (!changed) break;
+  }
+  free(x);
+  return sweeps;
+}
+
+/* ------------------------------------------------------- Pannotia BC
+ * bc_combined (src/pannotia/graph_apps/bc/device/bc.cl:186-235) for each
+ * source s: level-synchronous BFS accumulating path counts
+ * (mega_bfs_kernel_func, bc.cl:17-104), then the backward sweep
+ * accumulating dependencies and adding them to bc (mega_backtrack_kernel_func,
+ * bc.cl:106-183), which here pulls over out-edges instead of pushing over
+ * the reverse graph. The reference scales rho / sigma by 10000 in FP32;
+ * this is the unscaled FP64 form of the same sums:
+ *   sigma[w] = sum over in-edges (u, w) with d[u] = d[w] - 1 of sigma[u]
+ *   delta[v] = sum over out-edges (v, w) with d[w] = d[v] + 1 of
+ *              sigma[v] / sigma[w] * (1 + delta[w])
+ *   bc[v]   += delta[v] for v != s. */
+int64_t or_bc(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* rrow, const int32_t* rcol,
+              int32_t num_sources, double* bc) {
+  if (n <= 0 || num_sources < 0 || num_sources > n) return -1;
+  int32_t* d = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* q = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* lvl = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 2));
+  double* sig = (double*)malloc(sizeof(double) * (size_t)n);
+  double* del = (double*)malloc(sizeof(double) * (size_t)n);
+  if (!d || !q || !lvl || !sig || !del) {
+    free(d), free(q), free(lvl), free(sig), free(del);
+    return -2;
+  }
+  for (int32_t v = 0; v < n; v++) bc[v] = 0.0;
+  int64_t levels_total = 0;
+  for (int32_t s = 0; s < num_sources; s++) {
+    for (int32_t v = 0; v < n; v++) {
+      d[v] = -1;
+      del[v] = 0.0;
+    }
+    d[s] = 0;
+    sig[s] = 1.0;
+    q[0] = s;
+    lvl[0] = 0;
+    lvl[1] = 1;
+    int32_t L = 0, tail = 1;
+    while (lvl[L] < lvl[L + 1]) {
+      for (int32_t i = lvl[L]; i < lvl[L + 1]; i++) {
+        const int32_t v = q[i];
+        for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++)
+          if (d[col[e]] < 0) {
+            d[col[e]] = L + 1;
+            q[tail++] = col[e];
+          }
+      }
+      lvl[L + 2] = tail;
+      for (int32_t i = lvl[L + 1]; i < tail; i++) {
+        const int32_t w = q[i];
+        double acc = 0.0;
+        for (int32_t j = rrow[w]; j < rrow[w + 1]; j++)
+          if (d[rcol[j]] == L) acc += sig[rcol[j]];
+        sig[w] = acc;
+      }
+      L++;
+    }
+    levels_total += L;
+    for (int32_t l = L - 1; l >= 0; l--)
+      for (int32_t i = lvl[l]; i < lvl[l + 1]; i++) {
+        const int32_t v = q[i];
+        double acc = 0.0;
+        for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) {
+          const int32_t w = col[e];
+          if (d[w] == l + 1) acc += sig[v] / sig[w] * (1.0 + del[w]);
+        }
+        del[v] = acc;
+      }
+    for (int32_t v = 0; v < n; v++)
+      if (v != s) bc[v] += del[v];
+  }
+  free(d), free(q), free(lvl), free(sig), free(del);
+  return levels_total;
+}
+
 uint64_t or_fnv1a64(const void* data, int64_t len) {
   const unsigned char* p = (const unsigned char*)data;
   uint64_t h = 0xcbf29ce484222325ull;

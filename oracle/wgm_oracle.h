@@ -52,6 +52,23 @@ int or_bfs_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, int32_
 int or_sssp_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* wt, int32_t src,
                     uint32_t* dist, int32_t threads);
 
+/* In-edge (transposed) CSR: the in-edges of w listed by ascending source,
+ * each source's edges in CSR order; rwt may be NULL (then wt is ignored). */
+int or_reverse_csr(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* wt, int32_t* rrow,
+                   int32_t* rcol, int32_t* rwt);
+/* Pannotia SSSP: Jacobi min-plus sweeps over in-edges until nothing changes;
+ * x[src] = 0, others start at OR_BIG_NUM. Returns the sweeps, including the
+ * last one that changed nothing. */
+#define OR_BIG_NUM 99999999
+int or_sssp_minplus(int32_t n, const int32_t* rrow, const int32_t* rcol, const int32_t* rwt, int32_t src,
+                    int32_t* dist);
+/* Pannotia BC (Brandes) in FP64 over sources [0, num_sources): levels by
+ * BFS, path counts pulled over in-edges (reverse-CSR order), dependencies
+ * pulled over out-edges (CSR order), bc[v] += delta_s(v) for v != s with
+ * sources ascending. Returns the total number of BFS levels. */
+int64_t or_bc(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* rrow, const int32_t* rcol,
+              int32_t num_sources, double* bc);
+
 /* FNV-1a 64-bit digest of a byte range (fixture hashes). */
 uint64_t or_fnv1a64(const void* data, int64_t len);
 

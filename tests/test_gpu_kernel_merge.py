@@ -1,0 +1,140 @@
+"""GPU: kernels emitted by the HIP kernel_merge from stub-style sources
+(tests/kernel_merge/*.hip -> gen/, built into libwgm_kmerge.so) and run
+through libwgm: the standalone forms under wgm_launch_standalone, and the
+merged mega-kernels under the resizing scheduler while tasks take
+workgroups away (kills) and the persistent task takes them back (forks)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from gpu_workgroup_managment_amd import _lib as L
+from gpu_workgroup_managment_amd import scheduler as S
+
+pytestmark = pytest.mark.gpu
+os.environ.setdefault("WGM_HOST_TIMEOUT_S", "30")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KERNELS = ("km_mega_barrier", "km_mega_chunks", "km_standalone_barrier", "km_standalone_chunks")
+
+
+@pytest.fixture(scope="module")
+def km():
+    path = os.path.join(HERE, "kernel_merge", "libwgm_kmerge.so")
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for f in KERNELS:
+        getattr(lib, f).restype = ctypes.c_void_p
+    return lib
+
+
+def chunk_expected(inp, mult):
+    """chunk_loop's element function: 16 steps of x*1664525+1013904223, then x*mult+1 (mod 2^32)."""
+    x = inp.astype(np.uint64)
+    for _ in range(16):
+        x = (x * 1664525 + 1013904223) & 0xFFFFFFFF
+    return ((x * mult + 1) & 0xFFFFFFFF).astype(np.uint32)
+
+
+class Chunks:
+    def __init__(self, dev, n=1 << 20, chunk=1024, rounds=8, mult=2654435761, seed=3):
+        rng = np.random.default_rng(seed)
+        self.n, self.chunk, self.rounds, self.mult = n, chunk, rounds, mult
+        self.inp = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        self.total = rounds * ((n + chunk - 1) // chunk)
+        self.d_in, self.d_out = dev(self.inp), dev(np.zeros(n, np.uint32))
+        self.d_hits, self.d_next = dev(np.zeros(self.total, np.int32)), dev(np.full(1, 12345, np.int32))
+
+    def args(self):
+        return [self.d_in.ptr, self.d_out.ptr, self.d_hits.ptr, self.d_next.ptr, ctypes.c_int32(self.n),
+                ctypes.c_int32(self.chunk), ctypes.c_int32(self.rounds), ctypes.c_uint32(self.mult)]
+
+    def check(self):
+        out, hits = self.d_out.get(), self.d_hits.get()
+        exp = chunk_expected(self.inp, self.mult)
+        bad = np.flatnonzero(out != exp)
+        assert bad.size == 0, f"{bad.size} wrong outputs, first at {bad[0]}: {out[bad[0]]} != {exp[bad[0]]}"
+        assert np.all(hits == 1), f"chunks claimed != once: {np.unique(hits, return_counts=True)}"
+        assert self.d_next.get()[0] >= self.total  # the initialisation reset the counter before any claim
+
+    def free(self):
+        for d in (self.d_in, self.d_out, self.d_hits, self.d_next):
+            d.free()
+
+
+def test_emitted_standalone_barrier(rt, km, dev):
+    iters = dev(np.zeros(1, np.int32))
+    st = rt.launch_standalone(km.km_standalone_barrier(), L.Args([iters.ptr, ctypes.c_int32(500)]), wg_size=256)
+    print(st)
+    assert st["participating"] > 1
+    assert iters.get()[0] == 500
+    iters.free()
+
+
+@pytest.mark.parametrize("wg", [256, 1024])
+def test_emitted_standalone_work_queue(rt, km, dev, wg):
+    c = Chunks(dev)
+    st = rt.launch_standalone(km.km_standalone_chunks(), L.Args(c.args()), wg_size=wg)
+    print(wg, st)
+    assert st["participating"] > 1
+    c.check()
+    c.free()
+
+
+def reduce_buffers(dev, length=1 << 20):
+    buf = np.random.default_rng(7).integers(1, 1 << 30, length).astype(np.int32)
+    buf[654321] = 9
+    return length, dev(buf), dev(np.full(1, 2**31 - 1, np.int32)), dev(np.zeros(1, np.int32))
+
+
+def drive(m, p, max_tasks=30):
+    """Send tasks of 1..p-1 workgroups while the persistent task runs (at least one)."""
+    sizes = []
+    while len(sizes) < max_tasks and (not sizes or m.persistent_running()):
+        k = 1 + len(sizes) % (p - 1)
+        m.send_task_custom(k)
+        sizes.append(k)
+    m.wait_persistent(60)
+    return sizes
+
+
+def test_emitted_mega_barrier(rt, km, dev):
+    length, d_buf, d_res, d_calls = reduce_buffers(dev)
+    d_iters = dev(np.zeros(1, np.int32))
+    target = 20000
+    args = L.Args([ctypes.c_int32(length), d_buf.ptr, d_res.ptr, d_calls.ptr, d_iters.ptr, ctypes.c_int32(target)])
+    with S.Mega(rt, np_kind=S.NP_REDUCE_MIN) as m:
+        p = m.launch_custom(km.km_mega_barrier(), args, wg_size=256, num_wgs=16)
+        assert p == 15
+        m.send_persistent_custom(p)
+        sizes = drive(m, p)
+        m.quit()
+    print(len(sizes))
+    assert d_iters.get()[0] == target  # resumed from its restoration context across every resize
+    assert d_res.get()[0] == 9
+    assert d_calls.get()[0] == sum(sizes)  # every task ran on exactly its k workgroups
+    for d in (d_buf, d_res, d_calls, d_iters):
+        d.free()
+
+
+def test_emitted_mega_work_queue(rt, km, dev):
+    length, d_buf, d_res, d_calls = reduce_buffers(dev)
+    c = Chunks(dev, rounds=64)
+    args = L.Args([ctypes.c_int32(length), d_buf.ptr, d_res.ptr, d_calls.ptr] + c.args())
+    with S.Mega(rt, np_kind=S.NP_REDUCE_MIN) as m:
+        p = m.launch_custom(km.km_mega_chunks(), args, wg_size=256, num_wgs=16)
+        assert p == 15
+        m.send_persistent_custom(p)
+        sizes = drive(m, p)
+        m.quit()
+    print(len(sizes))
+    # forked workgroups skip the initialisation: their `scale` comes from the
+    # restoration context, so a lost context shows as wrong outputs
+    c.check()
+    assert d_res.get()[0] == 9
+    assert d_calls.get()[0] == sum(sizes)
+    c.free()
+    for d in (d_buf, d_res, d_calls):
+        d.free()

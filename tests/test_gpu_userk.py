@@ -16,44 +16,10 @@ from gpu_workgroup_managment_amd import scheduler as S
 pytestmark = pytest.mark.gpu
 os.environ.setdefault("WGM_HOST_TIMEOUT_S", "30")
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-
-
-@pytest.fixture(scope="module")
-def uk():
-    path = os.path.join(HERE, "kernels", "libwgm_userk.so")
-    if not os.path.exists(path):
-        raise ImportError(f"{path} missing: build it with __graft_entry__.build()")
-    lib = ctypes.CDLL(path)
-    for f in ("userk_rounds", "userk_mega"):
-        getattr(lib, f).restype = ctypes.c_void_p
-    lib.userk_alloc.argtypes = [ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]
-    lib.userk_free.argtypes = [ctypes.c_void_p]
-    lib.userk_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
-    lib.userk_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
-    return lib
-
-
-class Dev:
-    def __init__(self, uk, host):
-        self.uk, self.host = uk, np.ascontiguousarray(host)
-        self.ptr = ctypes.c_void_p()
-        assert uk.userk_alloc(self.host.nbytes, ctypes.byref(self.ptr)) == 0
-        assert uk.userk_h2d(self.ptr, self.host.ctypes.data, self.host.nbytes) == 0
-
-    def get(self):
-        out = np.empty_like(self.host)
-        assert self.uk.userk_d2h(out.ctypes.data, self.ptr, out.nbytes) == 0
-        return out
-
-    def free(self):
-        self.uk.userk_free(self.ptr)
-
-
 @pytest.mark.parametrize("wg,nw", [(256, 0), (1024, 0), (64, 24)])
-def test_standalone_user_kernel(rt, uk, wg, nw):
-    slots = Dev(uk, np.zeros(8192, np.int32))
-    out = Dev(uk, np.zeros(4, np.int32))
+def test_standalone_user_kernel(rt, uk, dev, wg, nw):
+    slots = dev(np.zeros(8192, np.int32))
+    out = dev(np.zeros(4, np.int32))
     args = L.Args([slots.ptr, out.ptr, ctypes.c_int32(20)])
     st = rt.launch_standalone(uk.userk_rounds(), args, wg_size=wg, num_wgs=nw)
     o = out.get()
@@ -67,15 +33,15 @@ def test_standalone_user_kernel(rt, uk, wg, nw):
     out.free()
 
 
-def test_merged_user_mega_kernel(rt, uk):
+def test_merged_user_mega_kernel(rt, uk, dev):
     length = 1 << 20
     rng = np.random.default_rng(7)
     buf = rng.integers(1, 1 << 30, length).astype(np.int32)
     buf[123457] = 5
-    d_buf = Dev(uk, buf)
-    d_res = Dev(uk, np.full(1, 2**31 - 1, np.int32))
-    d_calls = Dev(uk, np.zeros(1, np.int32))
-    d_iters = Dev(uk, np.zeros(1, np.int32))
+    d_buf = dev(buf)
+    d_res = dev(np.full(1, 2**31 - 1, np.int32))
+    d_calls = dev(np.zeros(1, np.int32))
+    d_iters = dev(np.zeros(1, np.int32))
     target = 20000
     args = L.Args([ctypes.c_int32(length), d_buf.ptr, d_res.ptr, d_calls.ptr, d_iters.ptr, ctypes.c_int32(target)])
     with S.Mega(rt, np_kind=S.NP_REDUCE_MIN) as m:
