@@ -238,6 +238,33 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
          {"edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
           "relaxations_scanned": best[4][1]["edges_scanned"]},
          cpu, f"threaded frontier Bellman-Ford, {threads} threads")
+    # Pannotia SSSP (min-plus sweeps over in-edges) on the same graph
+    t0 = time.perf_counter()
+    mref, jacobi = oracle.sssp_minplus(g.row_ptr, g.col, g.wt, 0)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.sssp_minplus(0, **lk))
+    sweeps = best[4][1]["iterations"]
+    indeg = np.bincount(g.col, minlength=g.n)
+    lanes = 4 if g.m < 8 * g.n else 16 if g.m < 32 * g.n else 64
+    ntasks = int(((indeg + 16 * lanes - 1) // (16 * lanes)).sum())
+    # per sweep: the task list, source + weight of every in-edge, one dist gather per in-edge
+    line("sssp_minplus", "Pannotia SSSP, RMAT scale 20 ef 16, w in [1,255], source 0", best,
+         sweeps * (16 * ntasks + 12 * g.m), bool(np.array_equal(best[4][0], mref)),
+         {"sweeps": sweeps, "jacobi_sweeps_oracle": jacobi, "tasks": ntasks,
+          "gteps_sweep_edges": round(sweeps * g.m / (best[0] * 1e6), 4)},
+         cpu, "serial Jacobi restatement")
+    dg.close()
+    # Pannotia BC, all sources
+    g = G.rmat(12, 8)
+    dg = rt.graph(g)
+    t0 = time.perf_counter()
+    bref, blevels = oracle.bc(g.row_ptr, g.col)
+    cpu = time.perf_counter() - t0
+    best = timed(lambda **lk: dg.bc(0, **lk))
+    # per source: out-edges twice (BFS 8 B, dependency pull 16 B), in-edges once (8 B), 28 B per vertex
+    line("bc", "Pannotia BC, RMAT scale 12 ef 8, all 4096 sources, FP64", best,
+         g.n * (24 * g.m + 8 * g.m + 28 * g.n), bool(np.array_equal(best[4][0], bref)),
+         {"levels": blevels, "gteps": round(g.n * g.m / (best[0] * 1e6), 4)}, cpu, "serial FP64 Brandes")
     dg.close()
     # color / MIS / PageRank on the grid
     g = G.grid_config()

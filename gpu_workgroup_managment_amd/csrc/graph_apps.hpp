@@ -153,10 +153,84 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   __syncthreads();
 }
 
+// frontier_batch for graphs whose out-degree never exceeds K (grids, meshes,
+// road-like graphs): K threads per frontier entry, one edge each, entries
+// in_q[a, b) with b - a <= blockDim.x / K. No degree scan, no edge-to-entry
+// search and no barrier before the relaxations: on such graphs a level is a
+// chain of dependent memory round trips, and the balanced form's three extra
+// barriers and LDS search lengthen every level of a high-diameter traversal.
+// Same relaxation rules and pushes as frontier_batch.
+template <bool SSSP, bool LOCAL, int K, bool COH = false>
+__device__ __forceinline__ void frontier_batch_low(const GraphDev& g, const FrontierDev& s, const int32_t* in_q,
+                                                   int a, int b, int32_t* out_q, int* out_cnt, uint32_t next,
+                                                   uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
+                                                   int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
+  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
+  const int i = a + (int)threadIdx.x / K, k = (int)threadIdx.x % K;
+  bool want = false, far_want = false;
+  int u = 0;
+  if (i < b) {
+    const int v = COH ? __hip_atomic_load(&in_q[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_q[i];
+    const int e = g.row_ptr[v] + k;
+    if (e < g.row_ptr[v + 1]) {
+      u = g.col[e];
+      if (!SSSP) {
+        if (s.dist[u] == kInf) {
+          uint32_t expect = kInf;
+          want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                      scope);
+        }
+      } else {
+        const uint32_t du = COH ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : s.dist[v];
+        const uint32_t alt = du + (uint32_t)g.wt[e];
+        if (alt < s.dist[u]) {
+          const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
+          if (alt < old) {
+            if (alt < thr) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+            else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
+          }
+        }
+      }
+    }
+  }
+  const unsigned long long mask = __ballot(want);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
+    base = __shfl(base, leader, 64);
+    if (want) {
+      int32_t* slot = &out_q[base + __popcll(mask & ((1ull << lane) - 1ull))];
+      if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else *slot = u;
+    }
+  }
+  if (SSSP && far_q) {
+    const unsigned long long fmask = __ballot(far_want);
+    if (fmask) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)fmask) - 1;
+      int base = 0;
+      if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
+      base = __shfl(base, leader, 64);
+      if (far_want) {
+        int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
+        if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *slot = u;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // Near-far state of one traversal: two far queues (n + 1 entries each) and
 // their counters, the in_far flags, the band minimum and width.
 struct NearFar {
   int32_t* q[2];
+  // select, not index: a runtime index into q[] puts the pair in scratch
+  __device__ __forceinline__ int32_t* far(int i) const { return i ? q[1] : q[0]; }
   int* cnt;          // lines: [0] count of q[0], [kLine] count of q[1], [2 kLine] band minimum
   uint32_t* flag;    // in_far[n]
   uint32_t delta;    // band width; 0: plain frontier Bellman-Ford
