@@ -38,12 +38,14 @@ constexpr long long kLowDegreeSingle = 1ll << 22;
 struct BatchJob {
   long long row_off;  // into the row_ptr pool
   long long col_off;  // into the col / wt pools
-  long long v_off;    // into the per-vertex pools (dist, stamp)
-  long long q_off;    // into the queue pool (2 * (n + 1) entries)
+  long long v_off;    // into the per-vertex pools (dist, in_far)
+  long long q_off;    // into the queue pools: two queues of qcap entries (q and, for SSSP, distance tags qa)
   long long f_off;    // into the far-queue pool (2 * (n + 1) entries) and the in_far pool (n)
   int n, m, query, team, src, ticket;
   int index;          // caller's graph index
   int maxdeg;         // largest out-degree (picks the degree-bounded frontier step)
+  int qcap;           // entries per queue: n + 1 (BFS), max(n, m) + 1 (SSSP, one entry per relaxation)
+  int pad;
 };
 
 struct BatchDev {
@@ -54,8 +56,9 @@ struct BatchDev {
   const int32_t* col;
   const int32_t* wt;
   uint32_t* dist;
-  uint32_t* stamp;
   int32_t* q;
+  uint32_t* qa;    // distance tags of SSSP queue entries (same offsets as q)
+  uint32_t* qr;    // row_info of queue entries (low-degree jobs)
   int* counts;     // 3 lines per job
   int* claims;     // 3 lines per job
   int chunk;
@@ -69,9 +72,11 @@ struct BatchDev {
   uint32_t delta;  // near-far band width (0: off)
 };
 
-// Wave-aggregated push of `item` where `want`, one counter atomic per wave.
+// Wave-aggregated push of `item` (and its distance tag into qa, when given)
+// where `want`, one counter atomic per wave.
 template <bool COH>
-__device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* cnt, int scope) {
+__device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* cnt, int scope,
+                                          uint32_t* qa = nullptr, uint32_t tag = 0) {
   const unsigned long long mask = __ballot(want);
   if (!mask) return;
   const int lane = threadIdx.x & 63;
@@ -84,9 +89,14 @@ __device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* 
   }
   base = __shfl(base, leader, 64);
   if (want) {
-    int32_t* slot = &q[base + __popcll(mask & ((1ull << lane) - 1ull))];
-    if (COH) __hip_atomic_store(slot, item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else *slot = item;
+    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+    if (COH) {
+      __hip_atomic_store(&q[pos], item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (qa) __hip_atomic_store(&qa[pos], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      q[pos] = item;
+      if (qa) qa[pos] = tag;
+    }
   }
 }
 
@@ -123,18 +133,18 @@ __device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epo
 // SSSP with nf.delta > 0 runs near-far bands: levels relax only below the
 // band's threshold; when the near frontier empties, the threshold moves to
 // the smallest deferred distance + delta and the far queue is split.
-template <bool SSSP, int K>
+template <bool SSSP>
 __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
                            int* iters_out) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
-    if (SSSP) s.stamp[v] = 0u;
     if (near_far) nf.flag[v] = 0u;
   }
   if (threadIdx.x == 0) {
     s.q0[0] = src;
+    if (SSSP) s.qa0[0] = 0u;
     s_cnt[0] = 1;
     s_cnt[1] = 0;
     s_cnt[2] = 0;
@@ -152,19 +162,12 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
       if (in_n == 0) break;
       const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
       int32_t* out_q = (level & 1) ? s.q0 : s.q1;
-      if (K > 0) {
-        constexpr int KK = K > 0 ? K : 1;
-        const int per = bd / KK;  // frontier entries per pass, K threads each
-        for (int a = 0; a < in_n; a += per)
-          frontier_batch_low<SSSP, true, KK>(g, s, in_q, a, min(a + per, in_n), out_q, &s_cnt[(level + 1) % 3],
-                                             (uint32_t)level + 1u, thr, near_far ? nf.far(fi) : nullptr,
-                                             &s_far[fi], nf.flag);
-      } else {
-        for (int a = 0; a < in_n; a += bd)
-          frontier_batch<SSSP, true>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
-                                     (uint32_t)level + 1u, lds, thr, near_far ? nf.far(fi) : nullptr, &s_far[fi],
-                                     nf.flag);
-      }
+      const uint32_t* in_qa = (level & 1) ? s.qa1 : s.qa0;
+      uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
+      for (int a = 0; a < in_n; a += bd)
+        frontier_batch<SSSP, true, false, SSSP>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
+                                                (uint32_t)level + 1u, lds, thr, near_far ? nf.far(fi) : nullptr,
+                                                &s_far[fi], nf.flag, in_qa, out_qa);
       if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
       __syncthreads();
     }
@@ -178,21 +181,186 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
     __syncthreads();
     thr = (uint32_t)s_far[2] + nf.delta;
     int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
     for (int base = 0; base < fn; base += bd) {
       const int i = base + threadIdx.x;
       bool to_near = false, to_far = false;
       int v = 0;
+      uint32_t dv = 0;
       if (i < fn) {
         v = nf.far(fi)[i];
-        if (s.dist[v] < thr) {
-          nf.flag[v] = 0u;
-          to_near = __hip_atomic_fetch_max(&s.stamp[v], (uint32_t)level, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP) < (uint32_t)level;
+        dv = s.dist[v];
+        if (dv < thr) {
+          nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
+          to_near = true;
         } else {
           to_far = true;  // stays flagged in the other far queue
         }
       }
-      push_item<false>(to_near, v, lq, &s_cnt[level % 3], __HIP_MEMORY_SCOPE_WORKGROUP);
+      push_item<false>(to_near, v, lq, &s_cnt[level % 3], __HIP_MEMORY_SCOPE_WORKGROUP, lqa, dv);
+      push_item<false>(to_far, v, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_far[fi] = 0;
+      s_far[2] = INT32_MAX;
+    }
+    fi = 1 - fi;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *iters_out = level;
+}
+
+// Row start and out-degree of v packed into one queue word (row start
+// < 2^29, degree <= 7), loaded by the producer alongside its atomic so the
+// consumer skips a dependent round trip.
+__device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
+  const int rs = g.row_ptr[v];
+  return ((uint32_t)rs << 3) | (uint32_t)(g.row_ptr[v + 1] - rs);
+}
+
+// A low-degree graph (out-degree <= K) on one workgroup: K threads per
+// frontier entry, one edge each, and entries that carry what the consumer
+// would otherwise fetch in sequence: the vertex, its row (row_info) and, for
+// SSSP, the distance it was queued with. A level is then three dependent
+// round trips -- entry; edge (+ the vertex's current distance); atomic (+ the
+// target's row) -- and one barrier. An SSSP entry whose vertex has since
+// dropped lower is skipped (its newer entry carries the lower distance).
+// Near-far bands as in run_single.
+template <bool SSSP, int K>
+__device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* s_cnt,
+                        int* iters_out) {
+  const bool near_far = SSSP && nf.delta > 0;
+  __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
+  for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
+    s.dist[v] = (v == src) ? 0u : kInf;
+    if (near_far) nf.flag[v] = 0u;
+  }
+  if (threadIdx.x == 0) {
+    s.q0[0] = src;
+    s.qr0[0] = row_info(g, src);
+    if (SSSP) s.qa0[0] = 0u;
+    s_cnt[0] = 1;
+    s_cnt[1] = 0;
+    s_cnt[2] = 0;
+    s_far[0] = 0;
+    s_far[1] = 0;
+    s_far[2] = INT32_MAX;
+  }
+  __syncthreads();
+  const int bd = blockDim.x, per = bd / K;
+  const int slot = (int)threadIdx.x / K, k = (int)threadIdx.x % K;
+  int level = 0, fi = 0;
+  uint32_t thr = near_far ? nf.delta : 0xffffffffu;
+  while (true) {
+    for (;; level++) {
+      const int in_n = __builtin_amdgcn_readfirstlane(s_cnt[level % 3]);
+      if (in_n == 0) break;
+      // counter (level + 2) % 3 was last read at the start of level - 1
+      if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
+      const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
+      int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+      const uint32_t* in_qa = (level & 1) ? s.qa1 : s.qa0;
+      uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
+      const uint32_t* in_qr = (level & 1) ? s.qr1 : s.qr0;
+      uint32_t* out_qr = (level & 1) ? s.qr0 : s.qr1;
+      int* out_cnt = &s_cnt[(level + 1) % 3];
+      const uint32_t next = (uint32_t)level + 1u;
+      for (int a = 0; a < in_n; a += per) {
+        const int i = a + slot;
+        bool want = false, far_want = false;
+        int u = 0;
+        uint32_t alt = next, ru = 0;
+        if (i < in_n) {
+          const uint32_t ri = in_qr[i];
+          const int v = in_q[i];
+          const uint32_t du = SSSP ? in_qa[i] : 0u;
+          if ((int)(ri & 7u) > k) {
+            const int e = (int)(ri >> 3) + k;
+            u = g.col[e];
+            if (!SSSP) {
+              ru = row_info(g, u);
+              uint32_t expect = kInf;
+              want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED,
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+              alt = du + (uint32_t)g.wt[e];
+              if (s.dist[v] == du) {  // live entry
+                ru = row_info(g, u);
+                const uint32_t old =
+                    __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (alt < old) {
+                  if (alt < thr) want = true;
+                  else far_want = __hip_atomic_exchange(&nf.flag[u], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+                }
+              }
+            }
+          }
+        }
+        const unsigned long long mask = __ballot(want);
+        if (mask) {
+          const int lane = threadIdx.x & 63;
+          const int leader = __ffsll((long long)mask) - 1;
+          int base = 0;
+          if (lane == leader)
+            base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          base = __shfl(base, leader, 64);
+          if (want) {
+            const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+            out_q[pos] = u;
+            out_qr[pos] = ru;
+            if (SSSP) out_qa[pos] = alt;
+          }
+        }
+        if (SSSP) push_item<false>(far_want, u, nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      __syncthreads();
+    }
+    if (!near_far) break;
+    const int fn = s_far[fi];
+    if (fn == 0) break;
+    int mn = INT32_MAX;
+    for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.far(fi)[i]]);
+    mn = wave_min_int(mn);
+    if ((threadIdx.x & 63) == 0) atomicMin(&s_far[2], mn);
+    __syncthreads();
+    thr = (uint32_t)s_far[2] + nf.delta;
+    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
+    uint32_t* lqr = (level & 1) ? s.qr1 : s.qr0;
+    for (int base = 0; base < fn; base += bd) {
+      const int i = base + threadIdx.x;
+      bool to_near = false, to_far = false;
+      int v = 0;
+      uint32_t dv = 0, rv = 0;
+      if (i < fn) {
+        v = nf.far(fi)[i];
+        dv = s.dist[v];
+        rv = row_info(g, v);
+        if (dv < thr) {
+          nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
+          to_near = true;
+        } else {
+          to_far = true;  // stays flagged in the other far queue
+        }
+      }
+      const unsigned long long mask = __ballot(to_near);
+      if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)mask) - 1;
+        int nb = 0;
+        if (lane == leader)
+          nb = __hip_atomic_fetch_add(&s_cnt[level % 3], __popcll(mask), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+        nb = __shfl(nb, leader, 64);
+        if (to_near) {
+          const int pos = nb + __popcll(mask & ((1ull << lane) - 1ull));
+          lq[pos] = v;
+          lqa[pos] = dv;
+          lqr[pos] = rv;
+        }
+      }
       push_item<false>(to_far, v, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
@@ -218,11 +386,11 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
   // initial state through the coherent path (other members' atomics see it)
   for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
     __hip_atomic_store(&s.dist[v], (v == src) ? 0u : kInf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (SSSP) __hip_atomic_store(&s.stamp[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (near_far) __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (rank == 0 && threadIdx.x == 0) {
     __hip_atomic_store(&s.q0[0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (SSSP) __hip_atomic_store(&s.qa0[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st_relaxed(&s.counts[0], 1);
     st_relaxed(&s.counts[kLine], 0);
     st_relaxed(&s.counts[2 * kLine], 0);
@@ -239,7 +407,7 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   while (true) {
     while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-      frontier_level<SSSP, true>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
+      frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
                                  nf.flag);
       team_barrier(tcnt, tgen, k, epoch);
       level++;
@@ -258,21 +426,23 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
     team_barrier(tcnt, tgen, k, epoch);
     thr = (uint32_t)ld_relaxed(&nf.cnt[2 * kLine]) + nf.delta;
     int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
     for (int base = rank * bd; base < fn; base += k * bd) {
       const int i = base + (int)threadIdx.x;
       bool to_near = false, to_far = false;
       int v = 0;
+      uint32_t dv = 0;
       if (i < fn) {
         v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < thr) {
+        dv = __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (dv < thr) {
           __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          to_near = __hip_atomic_fetch_max(&s.stamp[v], (uint32_t)level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                    (uint32_t)level;
+          to_near = true;
         } else {
           to_far = true;
         }
       }
-      push_item<true>(to_near, v, lq, &s.counts[(level % 3) * kLine], __HIP_MEMORY_SCOPE_AGENT);
+      push_item<true>(to_near, v, lq, &s.counts[(level % 3) * kLine], __HIP_MEMORY_SCOPE_AGENT, lqa, dv);
       push_item<true>(to_far, v, nf.far(1 - fi), &nf.cnt[(1 - fi) * kLine], __HIP_MEMORY_SCOPE_AGENT);
     }
     team_barrier(tcnt, tgen, k, epoch);
@@ -332,9 +502,13 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     g.m = jb.m;
     FrontierDev s{};
     s.dist = b.dist + jb.v_off;
-    s.stamp = b.stamp + jb.v_off;
+    s.stamp = nullptr;  // SSSP queues carry distance tags instead
     s.q0 = b.q + jb.q_off;
-    s.q1 = b.q + jb.q_off + jb.n + 1;
+    s.q1 = b.q + jb.q_off + jb.qcap;
+    s.qa0 = b.qa + jb.q_off;
+    s.qa1 = b.qa + jb.q_off + jb.qcap;
+    s.qr0 = b.qr + jb.q_off;
+    s.qr1 = b.qr + jb.q_off + jb.qcap;
     s.counts = b.counts + (size_t)j * 3 * kLine;
     s.claims = b.claims + (size_t)j * 3 * kLine;
     s.chunk = b.chunk;
@@ -349,12 +523,13 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     }
     if (jb.team <= 1) {
       // degree-bounded step for graphs whose out-degree never exceeds 4
-      if (jb.maxdeg <= kLowDegree) {
-        if (sssp) run_single<true, 4>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
-        else run_single<false, 4>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+      // (row starts must fit row_info's 29 bits)
+      if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
+        if (sssp) run_low<true, kLowDegree>(g, s, nf, jb.src, s_cnt, &b.iters[j]);
+        else run_low<false, kLowDegree>(g, s, nf, jb.src, s_cnt, &b.iters[j]);
       } else {
-        if (sssp) run_single<true, 0>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
-        else run_single<false, 0>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+        if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+        else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
       }
     } else {
       int* tcnt = b.tbar + (size_t)j * 2 * kLine;
@@ -408,7 +583,8 @@ struct wgm_batch {
   BatchJob* d_jobs_one = nullptr;  // relaunch baseline: the same jobs, tickets from 0
   int chunk = 0;
   int32_t *ticket = nullptr, *iters = nullptr;
-  uint32_t *dist = nullptr, *stamp = nullptr;
+  uint32_t *dist = nullptr, *qa = nullptr, *qr = nullptr;
+  std::vector<int32_t> qcap;      // entries per queue, per graph
   int32_t* far = nullptr;         // near-far queues, 2 (n + 1) per job (aligned)
   uint32_t* in_far = nullptr;     // near-far flags, per vertex (v_off)
   int32_t* fcnt = nullptr;        // 3 lines per job
@@ -454,7 +630,8 @@ BatchDev dev_view(wgm_batch* b) {
   d.col = b->col;
   d.wt = b->weighted ? b->wt : nullptr;
   d.dist = b->dist;
-  d.stamp = b->stamp;
+  d.qa = b->qa;
+  d.qr = b->qr;
   d.q = b->q;
   d.counts = b->counts;
   d.claims = b->claims;
@@ -509,6 +686,7 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.ticket = ticket;
     j.index = i;
     j.maxdeg = b->maxdeg[i];
+    j.qcap = b->qcap[i];
     ticket += j.team;
   }
   b->chunk = cfg.chunk;
@@ -619,12 +797,17 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     b->v_off.push_back(b->total_v);
     b->q_off.push_back(b->total_q);
     b->f_off.push_back(b->total_f);
+    // SSSP queues carry distance tags and a level queues at most one entry
+    // per relaxation; BFS claims each vertex once
+    const int64_t cap = (queries[i] == WGM_QUERY_SSSP ? std::max<int64_t>(g.n, g.m) : g.n) + 1;
+    if (cap > INT32_MAX / 2) return bail(fail(rt, WGM_ERR_INVALID, "graph %d: too large for the queue pools", i));
+    b->qcap.push_back((int32_t)cap);
     // per-job state on its own 256-byte lines: with team members reading
     // through their XCD's L2, no line may be shared with another job
     b->total_rows += g.n + 1;
     b->total_edges += g.m;
     b->total_v += ((int64_t)g.n + 63) / 64 * 64;
-    b->total_q += ((2 * ((int64_t)g.n + 1)) + 63) / 64 * 64;
+    b->total_q += (2 * cap + 63) / 64 * 64;
     b->total_f += ((2 * ((int64_t)g.n + 1)) + 63) / 64 * 64;
   }
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 4); };
@@ -632,8 +815,9 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->col, sizeof(int32_t) * b->total_edges);
   if (e == hipSuccess && b->weighted) e = alloc((void**)&b->wt, sizeof(int32_t) * b->total_edges);
   if (e == hipSuccess) e = alloc((void**)&b->dist, sizeof(uint32_t) * b->total_v);
-  if (e == hipSuccess) e = alloc((void**)&b->stamp, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = alloc((void**)&b->q, sizeof(int32_t) * b->total_q);
+  if (e == hipSuccess) e = alloc((void**)&b->qa, sizeof(uint32_t) * b->total_q);
+  if (e == hipSuccess) e = alloc((void**)&b->qr, sizeof(uint32_t) * b->total_q);
   if (e == hipSuccess) e = alloc((void**)&b->counts, sizeof(int32_t) * 3 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->claims, sizeof(int32_t) * 3 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs_one, sizeof(BatchJob) * count);
@@ -649,8 +833,9 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   // zeroed state: a queue slot read before it is written holds a vertex id of
   // its own job (0 at first), never garbage
   e = hipMemset(b->dist, 0, sizeof(uint32_t) * b->total_v);
-  if (e == hipSuccess) e = hipMemset(b->stamp, 0, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = hipMemset(b->q, 0, sizeof(int32_t) * b->total_q);
+  if (e == hipSuccess) e = hipMemset(b->qa, 0, sizeof(uint32_t) * b->total_q);
+  if (e == hipSuccess) e = hipMemset(b->qr, 0, sizeof(uint32_t) * b->total_q);
   if (e == hipSuccess) e = hipMemset(b->far, 0, sizeof(int32_t) * b->total_f);
   if (e == hipSuccess) e = hipMemset(b->in_far, 0, sizeof(uint32_t) * b->total_v);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMemset(batch state)"));
@@ -679,7 +864,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (!b) return WGM_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->stamp,
+  void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->qa, b->qr,
                   b->d_jobs, b->d_reached, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -40,6 +40,17 @@ struct FrontierDev {
   int* counts;      // 3 lines
   int* claims;      // 3 lines (dynamic chunk claiming), may be null
   int chunk;        // frontier vertices per claim; <= 0: static round-robin chunks
+  // SSSP with distance-tagged entries (PAIRS): qa[i] is the distance q[i] was
+  // queued with; an entry whose vertex has since dropped lower is stale and
+  // skipped (its newer entry carries the lower distance). One atomic per
+  // improving edge instead of two; a level queues at most one entry per
+  // relaxation, so each queue holds max(n, m) + 1 entries.
+  uint32_t* qa0;
+  uint32_t* qa1;
+  // row start << 3 | out-degree of each entry's vertex (the batched
+  // low-degree path, which loads it when it queues the entry)
+  uint32_t* qr0;
+  uint32_t* qr1;
 };
 
 // Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17], claim[1]
@@ -66,11 +77,12 @@ __host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 
 // once while its in_far flag is set; the caller processes it in a later
 // band. Final distances are the same (they are unique); the work is closer
 // to Dijkstra's order.
-template <bool SSSP, bool LOCAL, bool COH = false>
+template <bool SSSP, bool LOCAL, bool COH = false, bool PAIRS = false>
 __device__ __forceinline__ void frontier_batch(const GraphDev& g, const FrontierDev& s, const int32_t* in_q, int a,
                                                int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds,
                                                uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
-                                               int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
+                                               int* far_cnt = nullptr, uint32_t* in_far = nullptr,
+                                               const uint32_t* in_qa = nullptr, uint32_t* out_qa = nullptr) {
   constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
   const int bd = blockDim.x, tid = threadIdx.x;
   int* s_end = lds;
@@ -85,6 +97,10 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
     beg = g.row_ptr[v];
     deg = g.row_ptr[v + 1] - beg;
     if (SSSP) du = COH ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : s.dist[v];
+    if (PAIRS) {
+      const uint32_t tag = COH ? __hip_atomic_load(&in_qa[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_qa[i];
+      if (tag != du) deg = 0;  // stale: v was queued again with a lower distance
+    }
   }
   int total;
   const int excl = wg_exclusive_scan(deg, s_wave, &total);
@@ -96,6 +112,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
     const int e = eb + tid;
     bool want = false, far_want = false;
     int u = 0;
+    uint32_t alt = 0;
     if (e < total) {
       int lo = 0, hi = bd - 1;
       while (lo < hi) {
@@ -111,11 +128,11 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
                                                       scope);
         }
       } else {
-        const uint32_t alt = s_du[lo] + (uint32_t)g.wt[idx];
+        alt = s_du[lo] + (uint32_t)g.wt[idx];
         if (alt < s.dist[u]) {
           const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
           if (alt < old) {
-            if (alt < thr) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+            if (alt < thr) want = PAIRS || __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
             else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
           }
         }
@@ -129,9 +146,14 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
       if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
       base = __shfl(base, leader, 64);
       if (want) {
-        int32_t* slot = &out_q[base + __popcll(mask & ((1ull << lane) - 1ull))];
-        if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else *slot = u;
+        const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (COH) {
+          __hip_atomic_store(&out_q[pos], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (PAIRS) __hip_atomic_store(&out_qa[pos], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          out_q[pos] = u;
+          if (PAIRS) out_qa[pos] = alt;
+        }
       }
     }
     if (SSSP && far_q) {
@@ -153,78 +175,6 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   __syncthreads();
 }
 
-// frontier_batch for graphs whose out-degree never exceeds K (grids, meshes,
-// road-like graphs): K threads per frontier entry, one edge each, entries
-// in_q[a, b) with b - a <= blockDim.x / K. No degree scan, no edge-to-entry
-// search and no barrier before the relaxations: on such graphs a level is a
-// chain of dependent memory round trips, and the balanced form's three extra
-// barriers and LDS search lengthen every level of a high-diameter traversal.
-// Same relaxation rules and pushes as frontier_batch.
-template <bool SSSP, bool LOCAL, int K, bool COH = false>
-__device__ __forceinline__ void frontier_batch_low(const GraphDev& g, const FrontierDev& s, const int32_t* in_q,
-                                                   int a, int b, int32_t* out_q, int* out_cnt, uint32_t next,
-                                                   uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
-                                                   int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
-  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
-  const int i = a + (int)threadIdx.x / K, k = (int)threadIdx.x % K;
-  bool want = false, far_want = false;
-  int u = 0;
-  if (i < b) {
-    const int v = COH ? __hip_atomic_load(&in_q[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_q[i];
-    const int e = g.row_ptr[v] + k;
-    if (e < g.row_ptr[v + 1]) {
-      u = g.col[e];
-      if (!SSSP) {
-        if (s.dist[u] == kInf) {
-          uint32_t expect = kInf;
-          want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                      scope);
-        }
-      } else {
-        const uint32_t du = COH ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                : s.dist[v];
-        const uint32_t alt = du + (uint32_t)g.wt[e];
-        if (alt < s.dist[u]) {
-          const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
-          if (alt < old) {
-            if (alt < thr) want = __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
-            else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
-          }
-        }
-      }
-    }
-  }
-  const unsigned long long mask = __ballot(want);
-  if (mask) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
-    base = __shfl(base, leader, 64);
-    if (want) {
-      int32_t* slot = &out_q[base + __popcll(mask & ((1ull << lane) - 1ull))];
-      if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      else *slot = u;
-    }
-  }
-  if (SSSP && far_q) {
-    const unsigned long long fmask = __ballot(far_want);
-    if (fmask) {
-      const int lane = threadIdx.x & 63;
-      const int leader = __ffsll((long long)fmask) - 1;
-      int base = 0;
-      if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
-      base = __shfl(base, leader, 64);
-      if (far_want) {
-        int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
-        if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else *slot = u;
-      }
-    }
-  }
-  __syncthreads();
-}
-
 // Near-far state of one traversal: two far queues (n + 1 entries each) and
 // their counters, the in_far flags, the band minimum and width.
 struct NearFar {
@@ -240,7 +190,7 @@ struct NearFar {
 // workgroups. Static mode: chunks of bd entries round-robin over the team.
 // Dynamic mode (s.chunk > 0): workgroups claim s.chunk entries at a time from
 // a per-level counter, which balances uneven chunks.
-template <bool SSSP, bool COH = false>
+template <bool SSSP, bool COH = false, bool PAIRS = false>
 __device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
                                                int rank, int nteam, int* lds, uint32_t thr = 0xffffffffu,
                                                int32_t* far_q = nullptr, int* far_cnt = nullptr,
@@ -249,6 +199,8 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
   const int in_n = ld_relaxed(&s.counts[(level % 3) * kLine]);
   const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
   int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+  const uint32_t* in_qa = (level & 1) ? s.qa1 : s.qa0;
+  uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
   int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
   const uint32_t next = (uint32_t)level + 1u;
   if (s.chunk > 0 && s.claims) {
@@ -262,14 +214,14 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
       if (c0 >= in_n) break;
       const int c1 = min(c0 + s.chunk, in_n);
       for (int a = c0; a < c1; a += bd)
-        frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr, far_q,
-                                         far_cnt, in_far);
+        frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr,
+                                                far_q, far_cnt, in_far, in_qa, out_qa);
     }
     if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
   } else {
     for (int a = rank * bd; a < in_n; a += nteam * bd)
-      frontier_batch<SSSP, false, COH>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr, far_q,
-                                       far_cnt, in_far);
+      frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr,
+                                              far_q, far_cnt, in_far, in_qa, out_qa);
   }
   if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
 }
