@@ -128,7 +128,7 @@ class BatchConfig(ctypes.Structure):
         ("max_team", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("delta", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sssp_team_edges", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -295,6 +295,7 @@ SIGNATURES = {
     "wgm_batch_run": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
     "wgm_batch_run_relaunch": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
     "wgm_batch_result": (_i32, [_vp, _i32, _vp, _P(_i32), _P(_i64)]),
+    "wgm_batch_job_times": (_i32, [_vp, _vp, _vp]),
     "wgm_batch_autotune": (_i32, [_vp, _vp, _i32, _P(BatchConfig), _P(BatchStats)]),
 }
 

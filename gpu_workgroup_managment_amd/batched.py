@@ -9,11 +9,12 @@ from . import _lib as L
 from . import graphs as G
 
 
-def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False, delta=0):
+def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False, delta=0,
+           sssp_team_edges=0):
     c = L.BatchConfig()
     c.wg_size, c.wgs_per_cu, c.chunk = wg_size, wgs_per_cu, chunk
     c.team_edges, c.max_team, c.use_graph = team_edges, max_team, 1 if use_graph else 0
-    c.delta = delta
+    c.delta, c.sssp_team_edges = delta, sssp_team_edges
     return c
 
 
@@ -61,6 +62,12 @@ class Batch:
         it, edges = ctypes.c_int32(), ctypes.c_int64()
         self.rt._check(L.lib().wgm_batch_result(self._h, i, L.ptr(dist), ctypes.byref(it), ctypes.byref(edges)))
         return dist, it.value, edges.value
+
+    def job_times(self):
+        """(start_us, end_us) per query of the last run, from the first query's start."""
+        t0, t1 = np.empty(len(self.graphs)), np.empty(len(self.graphs))
+        self.rt._check(L.lib().wgm_batch_job_times(self._h, L.ptr(t0), L.ptr(t1)))
+        return t0, t1
 
     def autotune(self, reps=1):
         best, st = L.BatchConfig(), L.BatchStats()

@@ -34,6 +34,18 @@ namespace wgm {
 // teams for the 1M-edge grids the batch took 17.4 ms instead of 10.2 ms.
 constexpr int kLowDegree = 4;
 constexpr long long kLowDegreeSingle = 1ll << 22;
+constexpr int kLowSlots = 4;  // frontier entries per thread group in flight together
+
+// The low-degree step keeps the first low_lds_cap(bd) entries of each of its
+// two frontier queues (vertex, row info, distance tag) in LDS behind the
+// frontier step's own LDS; entries past that spill to the HBM queues at the
+// same index. A level then reads its entries from LDS and ends without
+// waiting on queue stores to reach L2. 2 * bd entries keep a 1024-thread
+// workgroup under 64 KB of LDS, and two such workgroups fit one CU.
+__host__ __device__ constexpr int low_lds_cap(int bd) { return 2 * bd; }
+__host__ __device__ constexpr int batch_lds_bytes(int bd) {
+  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd);
+}
 
 struct BatchJob {
   long long row_off;  // into the row_ptr pool
@@ -70,6 +82,7 @@ struct BatchDev {
   uint32_t* in_far;
   int* fcnt;       // 3 lines per job: far counts and band minimum
   uint32_t delta;  // near-far band width (0: off)
+  unsigned long long* jt;  // per job: first member's start, last member's end (wall clock)
 };
 
 // Wave-aggregated push of `item` (and its distance tag into qa, when given)
@@ -219,27 +232,68 @@ __device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
   return ((uint32_t)rs << 3) | (uint32_t)(g.row_ptr[v + 1] - rs);
 }
 
+// An empty register barrier: the compiler may not fold a later use of v back
+// into the conditional block that produced it, where it would wait for the
+// load or atomic before the next slot's is issued.
+__device__ __forceinline__ void opaque(uint32_t& v) { asm volatile("" : "+v"(v)); }
+
+// Frontier queues of the low-degree step: entry i of queue p (0 or 1) in
+// LDS below cap, in the HBM queue at index i above.
+struct LowQueues {
+  uint32_t* lds;  // [queue][vertex, row info, tag][cap]
+  int cap;
+  const FrontierDev* s;
+  __device__ __forceinline__ void load(int p, int i, int& v, uint32_t& ri, uint32_t& du, bool tag) const {
+    if (i < cap) {
+      const uint32_t* b = lds + p * 3 * cap;
+      v = (int)b[i];
+      ri = b[cap + i];
+      du = tag ? b[2 * cap + i] : 0u;
+    } else {
+      v = (p ? s->q1 : s->q0)[i];
+      ri = (p ? s->qr1 : s->qr0)[i];
+      du = tag ? (p ? s->qa1 : s->qa0)[i] : 0u;
+    }
+  }
+  __device__ __forceinline__ void store(int p, int i, int v, uint32_t ri, uint32_t du, bool tag) const {
+    if (i < cap) {
+      uint32_t* b = lds + p * 3 * cap;
+      b[i] = (uint32_t)v;
+      b[cap + i] = ri;
+      if (tag) b[2 * cap + i] = du;
+    } else {
+      (p ? s->q1 : s->q0)[i] = v;
+      (p ? s->qr1 : s->qr0)[i] = ri;
+      if (tag) (p ? s->qa1 : s->qa0)[i] = du;
+    }
+  }
+};
+
 // A low-degree graph (out-degree <= K) on one workgroup: K threads per
 // frontier entry, one edge each, and entries that carry what the consumer
 // would otherwise fetch in sequence: the vertex, its row (row_info) and, for
-// SSSP, the distance it was queued with. A level is then three dependent
-// round trips -- entry; edge (+ the vertex's current distance); atomic (+ the
-// target's row) -- and one barrier. An SSSP entry whose vertex has since
-// dropped lower is skipped (its newer entry carries the lower distance).
-// Near-far bands as in run_single.
-template <bool SSSP, int K>
-__device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* s_cnt,
+// SSSP, the distance it was queued with. With the queues in LDS a pass over
+// entries is two dependent HBM round trips -- edge (+ the vertex's current
+// distance); atomic (+ the target's row) -- and each thread keeps U entries'
+// round trips in flight together, so a level of up to U * bd / K entries is
+// one pass and one barrier. An SSSP entry whose vertex has since dropped
+// lower is skipped (its newer entry carries the lower distance). Near-far
+// bands as in run_single; a far queue of at most bd entries is split in one
+// pass that keeps each entry's distance and row in registers across the
+// band minimum.
+template <bool SSSP, int K, int U>
+__device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
                         int* iters_out) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
+  const LowQueues lq{reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4,
+                     low_lds_cap((int)blockDim.x), &s};
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
     if (near_far) nf.flag[v] = 0u;
   }
   if (threadIdx.x == 0) {
-    s.q0[0] = src;
-    s.qr0[0] = row_info(g, src);
-    if (SSSP) s.qa0[0] = 0u;
+    lq.store(0, 0, src, row_info(g, src), 0u, SSSP);
     s_cnt[0] = 1;
     s_cnt[1] = 0;
     s_cnt[2] = 0;
@@ -250,6 +304,8 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   __syncthreads();
   const int bd = blockDim.x, per = bd / K;
   const int slot = (int)threadIdx.x / K, k = (int)threadIdx.x % K;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
   int level = 0, fi = 0;
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   while (true) {
@@ -258,86 +314,98 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
       if (in_n == 0) break;
       // counter (level + 2) % 3 was last read at the start of level - 1
       if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
-      const int32_t* in_q = (level & 1) ? s.q1 : s.q0;
-      int32_t* out_q = (level & 1) ? s.q0 : s.q1;
-      const uint32_t* in_qa = (level & 1) ? s.qa1 : s.qa0;
-      uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
-      const uint32_t* in_qr = (level & 1) ? s.qr1 : s.qr0;
-      uint32_t* out_qr = (level & 1) ? s.qr0 : s.qr1;
+      const int p = level & 1;  // in queue; 1 - p is the out queue
       int* out_cnt = &s_cnt[(level + 1) % 3];
       const uint32_t next = (uint32_t)level + 1u;
-      for (int a = 0; a < in_n; a += per) {
-        const int i = a + slot;
-        bool want = false, far_want = false;
-        int u = 0;
-        uint32_t alt = next, ru = 0;
-        if (i < in_n) {
-          const uint32_t ri = in_qr[i];
-          const int v = in_q[i];
-          const uint32_t du = SSSP ? in_qa[i] : 0u;
-          if ((int)(ri & 7u) > k) {
-            const int e = (int)(ri >> 3) + k;
-            u = g.col[e];
-            if (!SSSP) {
-              ru = row_info(g, u);
-              uint32_t expect = kInf;
-              want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED,
-                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              alt = du + (uint32_t)g.wt[e];
-              if (s.dist[v] == du) {  // live entry
-                ru = row_info(g, u);
-                const uint32_t old =
-                    __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (alt < old) {
-                  if (alt < thr) want = true;
-                  else far_want = __hip_atomic_exchange(&nf.flag[u], 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
-                }
-              }
-            }
+      for (int a = 0; a < in_n; a += per * U) {
+        int v[U], e[U];
+        uint32_t du[U];
+        bool ok[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          const int i = a + x * per + slot;
+          uint32_t ri = 0;
+          v[x] = 0;
+          du[x] = 0;
+          if (i < in_n) lq.load(p, i, v[x], ri, du[x], SSSP);
+          ok[x] = (int)(ri & 7u) > k;
+          e[x] = (int)(ri >> 3) + k;
+        }
+        // round trip 1, every slot in flight: the edge and the entry's distance
+        int u[U];
+        uint32_t w[U], dv[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          u[x] = ok[x] ? g.col[e[x]] : 0;
+          w[x] = SSSP && ok[x] ? (uint32_t)g.wt[e[x]] : 0u;
+          dv[x] = SSSP && ok[x] ? s.dist[v[x]] : 0u;
+        }
+        // round trip 2, every slot in flight: the atomic and the target's row.
+        // Only the memory operations sit under each slot's condition; what
+        // reads their results comes after every slot has issued.
+        uint32_t alt[U], old[U], r0[U], r1[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          const bool live = ok[x] && (!SSSP || dv[x] == du[x]);
+          alt[x] = SSSP ? du[x] + w[x] : next;
+          // the row loads are unconditional (a dead slot reads row 0, which
+          // every graph has): only the atomic needs the slot's condition
+          const int uu = live ? u[x] : 0;
+          r0[x] = (uint32_t)g.row_ptr[uu];
+          r1[x] = (uint32_t)g.row_ptr[uu + 1];
+          old[x] = 0u;  // not live: never an improvement, never kInf
+          if (live) {
+            // BFS: every distance set so far is <= level + 1 = alt, so the
+            // minimum changes only kInf and old == kInf marks the discovery
+            // (a compare-and-swap without its compare inside the block)
+            old[x] = __hip_atomic_fetch_min(&s.dist[u[x]], alt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
-        const unsigned long long mask = __ballot(want);
-        if (mask) {
-          const int lane = threadIdx.x & 63;
-          const int leader = __ffsll((long long)mask) - 1;
-          int base = 0;
-          if (lane == leader)
-            base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          base = __shfl(base, leader, 64);
-          if (want) {
-            const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-            out_q[pos] = u;
-            out_qr[pos] = ru;
-            if (SSSP) out_qa[pos] = alt;
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          opaque(old[x]);
+          opaque(r0[x]);
+          opaque(r1[x]);
+        }
+        bool want[U], far_cand[U];
+        uint32_t ru[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          ru[x] = (r0[x] << 3) | (r1[x] - r0[x]);
+          const bool better = SSSP ? alt[x] < old[x] : old[x] == kInf;
+          want[x] = better && (!SSSP || alt[x] < thr);
+          far_cand[x] = SSSP && better && alt[x] >= thr;
+        }
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          const unsigned long long mask = __ballot(want[x]);
+          if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            int base = 0;
+            if (lane == leader)
+              base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            base = __shfl(base, leader, 64);
+            if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP);
+          }
+          if (SSSP) {
+            const bool far_want = far_cand[x] && __hip_atomic_exchange(&nf.flag[u[x]], 1u, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+            push_item<false>(far_want, u[x], nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
-        if (SSSP) push_item<false>(far_want, u, nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       __syncthreads();
     }
     if (!near_far) break;
-    const int fn = s_far[fi];
+    // uniform: read after the level's barrier (scalar, so the branches
+    // around the barriers below stay uniform for the compiler too)
+    const int fn = __builtin_amdgcn_readfirstlane(s_far[fi]);
     if (fn == 0) break;
-    int mn = INT32_MAX;
-    for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.far(fi)[i]]);
-    mn = wave_min_int(mn);
-    if ((threadIdx.x & 63) == 0) atomicMin(&s_far[2], mn);
-    __syncthreads();
-    thr = (uint32_t)s_far[2] + nf.delta;
-    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
-    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
-    uint32_t* lqr = (level & 1) ? s.qr1 : s.qr0;
-    for (int base = 0; base < fn; base += bd) {
-      const int i = base + threadIdx.x;
+    const int p = level & 1;  // this level's (empty) queue
+    // move a far entry below the new threshold to the near queue, else to the other far queue
+    auto split = [&](bool in, int v, uint32_t dv, uint32_t rv) {
       bool to_near = false, to_far = false;
-      int v = 0;
-      uint32_t dv = 0, rv = 0;
-      if (i < fn) {
-        v = nf.far(fi)[i];
-        dv = s.dist[v];
-        rv = row_info(g, v);
+      if (in) {
         if (dv < thr) {
           nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
           to_near = true;
@@ -347,21 +415,49 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
       }
       const unsigned long long mask = __ballot(to_near);
       if (mask) {
-        const int lane = threadIdx.x & 63;
         const int leader = __ffsll((long long)mask) - 1;
         int nb = 0;
         if (lane == leader)
           nb = __hip_atomic_fetch_add(&s_cnt[level % 3], __popcll(mask), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         nb = __shfl(nb, leader, 64);
-        if (to_near) {
-          const int pos = nb + __popcll(mask & ((1ull << lane) - 1ull));
-          lq[pos] = v;
-          lqa[pos] = dv;
-          lqr[pos] = rv;
-        }
+        if (to_near) lq.store(p, nb + __popcll(mask & below), v, rv, dv, true);
       }
       push_item<false>(to_far, v, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    if (fn <= bd) {
+      const bool in = (int)threadIdx.x < fn;
+      int v = 0;
+      uint32_t dv = 0, rv = 0;
+      if (in) {
+        v = nf.far(fi)[threadIdx.x];
+        dv = s.dist[v];
+        rv = row_info(g, v);
+      }
+      const int mn = wave_min_int(in ? (int)dv : INT32_MAX);
+      if (lane == 0) atomicMin(&s_far[2], mn);
+      __syncthreads();
+      thr = (uint32_t)__builtin_amdgcn_readfirstlane(s_far[2]) + nf.delta;
+      split(in, v, dv, rv);
+    } else {
+      int mn = INT32_MAX;
+      for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.far(fi)[i]]);
+      mn = wave_min_int(mn);
+      if (lane == 0) atomicMin(&s_far[2], mn);
+      __syncthreads();
+      thr = (uint32_t)__builtin_amdgcn_readfirstlane(s_far[2]) + nf.delta;
+      for (int base = 0; base < fn; base += bd) {
+        const int i = base + threadIdx.x;
+        const bool in = i < fn;
+        int v = 0;
+        uint32_t dv = 0, rv = 0;
+        if (in) {
+          v = nf.far(fi)[i];
+          dv = s.dist[v];
+          rv = row_info(g, v);
+        }
+        split(in, v, dv, rv);
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -485,6 +581,8 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
         }
         j = lo;
         r = t - b.jobs[j].ticket;
+        __hip_atomic_fetch_min(&b.jt[2 * j], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
       }
       s_job[0] = j;
       s_job[1] = r;
@@ -525,8 +623,8 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       // degree-bounded step for graphs whose out-degree never exceeds 4
       // (row starts must fit row_info's 29 bits)
       if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
-        if (sssp) run_low<true, kLowDegree>(g, s, nf, jb.src, s_cnt, &b.iters[j]);
-        else run_low<false, kLowDegree>(g, s, nf, jb.src, s_cnt, &b.iters[j]);
+        if (sssp) run_low<true, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+        else run_low<false, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
       } else {
         if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
         else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
@@ -537,6 +635,13 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       if (sssp) run_team<true>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
       else run_team<false>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
     }
+    // Thread 0 stamps the end before the loop's closing barrier: a thread-0
+    // section after it would sit next to the ticket claim across the back
+    // edge, and the two merged branches split the job loop into a divergent
+    // inner loop around the barriers (lanes of wave 0 then meet them apart).
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_max(&b.jt[2 * j + 1], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
   }
   // the last workgroup to leave stamps the end of the batch
@@ -550,7 +655,10 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
 // kernel, not a memset).
 __global__ void k_batch_reset(BatchDev b) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int i = t; i < b.njobs * 2; i += gridDim.x * blockDim.x) b.tbar[i * kLine] = 0;
+  for (int i = t; i < b.njobs * 2; i += gridDim.x * blockDim.x) {
+    b.tbar[i * kLine] = 0;
+    b.jt[i] = (i & 1) ? 0ull : ~0ull;
+  }
   if (t == 0) *b.ticket = 0;
 }
 
@@ -583,6 +691,10 @@ struct wgm_batch {
   BatchJob* d_jobs_one = nullptr;  // relaunch baseline: the same jobs, tickets from 0
   int chunk = 0;
   int32_t *ticket = nullptr, *iters = nullptr;
+  unsigned long long* jt = nullptr;     // per job: start, end (wall clock)
+  std::vector<double> start_us, end_us;  // per caller index, after a run
+  std::vector<double> dur_us;            // per caller index: last run's duration (empty before one)
+  bool jobs_by_time = false;             // job table ordered by measured durations
   uint32_t *dist = nullptr, *qa = nullptr, *qr = nullptr;
   std::vector<int32_t> qcap;      // entries per queue, per graph
   int32_t* far = nullptr;         // near-far queues, 2 (n + 1) per job (aligned)
@@ -607,9 +719,11 @@ struct wgm_batch {
 
 namespace {
 
-// Team size: ceil(m / team_edges), capped; see kLowDegreeSingle.
-int team_size(const wgm_batch_config& cfg, int64_t m, int32_t maxdeg) {
-  const int64_t te = cfg.team_edges > 0 ? cfg.team_edges : (1 << 16);
+// Team size: ceil(m / team_edges) (sssp_team_edges for SSSP queries), capped;
+// see kLowDegreeSingle.
+int team_size(const wgm_batch_config& cfg, int64_t m, int32_t maxdeg, bool sssp) {
+  int64_t te = cfg.team_edges > 0 ? cfg.team_edges : (1 << 16);
+  if (sssp && cfg.sssp_team_edges > 0) te = cfg.sssp_team_edges;
   int64_t k = maxdeg <= kLowDegree ? (m + kLowDegreeSingle - 1) / kLowDegreeSingle : (m + te - 1) / te;
   if (k < 1) k = 1;
   if (k > cfg.max_team) k = cfg.max_team;
@@ -639,6 +753,7 @@ BatchDev dev_view(wgm_batch* b) {
   d.tbar = b->tbar;
   d.ticket = b->ticket;
   d.iters = b->iters;
+  d.jt = b->jt;
   d.far = b->far;
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
@@ -655,7 +770,7 @@ int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out
   if (c.team_edges <= 0) c.team_edges = 1 << 16;
   if (c.chunk <= 0) c.chunk = c.wg_size;
   if (c.delta < 0) c.delta = 0;
-  c.reserved = 0;
+  if (c.sssp_team_edges < 0) c.sssp_team_edges = 0;
   int rc = wgm_check_wg(rt, c.wg_size);
   if (rc) return rc;
   if (c.wgs_per_cu > 32 || c.max_team > 1024) return fail(rt, WGM_ERR_INVALID, "bad batch config");
@@ -663,11 +778,18 @@ int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out
   return WGM_OK;
 }
 
-// Job table for a config: LPT order (edges descending), team sizes, tickets.
+// Job table for a config: LPT order, team sizes, tickets. The order is by
+// each query's duration in the last run once one has been timed, by edge
+// count before: a long, narrow traversal (a large grid's SSSP: ~1500 levels
+// on one workgroup) outlasts a larger but shallow graph's team, and starts
+// first only when measured.
 int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
   std::vector<int> order(b->count);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->m[a] > b->m[c]; });
+  const bool timed = b->dur_us.size() == (size_t)b->count;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return timed ? b->dur_us[a] > b->dur_us[c] : b->m[a] > b->m[c]; });
+  b->jobs_by_time = timed;
   b->jobs.assign(b->count, BatchJob{});
   int ticket = 0;
   for (int k = 0; k < b->count; k++) {
@@ -681,7 +803,8 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.n = b->n[i];
     j.m = (int)b->m[i];
     j.query = b->query[i];
-    j.team = std::min(team_size(cfg, b->m[i], b->maxdeg[i]), std::max(1, grid));
+    const bool sssp = b->query[i] == WGM_QUERY_SSSP;
+    j.team = std::min(team_size(cfg, b->m[i], b->maxdeg[i], sssp), std::max(1, grid));
     j.src = b->src[i];
     j.ticket = ticket;
     j.index = i;
@@ -716,7 +839,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   int rc = wgm_enqueue_reset_ctl(rt, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
-  hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size), st, rt->ctl, d);
+  hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
   WGM_HIPCHK(rt, hipGetLastError());
   return WGM_OK;
 }
@@ -727,18 +850,36 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
   WGM_HIPCHK(rt, hipMemsetAsync(b->d_reached, 0, sizeof(unsigned long long) * b->count, rt->stream));
   hipLaunchKernelGGL(k_batch_reached, dim3(std::min(b->count, 1024)), dim3(256), 0, rt->stream, d, b->d_reached);
   WGM_HIPCHK(rt, hipGetLastError());
-  std::vector<unsigned long long> per_job(b->count);
+  std::vector<unsigned long long> per_job(b->count), jt(2 * (size_t)b->count);
   std::vector<int> iters(b->count);
   WGM_HIPCHK(rt, hipMemcpyAsync(per_job.data(), b->d_reached, sizeof(unsigned long long) * b->count,
                                 hipMemcpyDeviceToHost, rt->stream));
   WGM_HIPCHK(rt, hipMemcpyAsync(iters.data(), b->iters, sizeof(int) * b->count, hipMemcpyDeviceToHost, rt->stream));
+  WGM_HIPCHK(rt, hipMemcpyAsync(jt.data(), b->jt, sizeof(unsigned long long) * jt.size(), hipMemcpyDeviceToHost,
+                                rt->stream));
   WGM_HIPCHK(rt, hipStreamSynchronize(rt->stream));
+  unsigned long long t0 = ~0ull;
+  for (int k = 0; k < b->count; k++) t0 = std::min(t0, jt[2 * k]);
+  const double us_per_tick = 1e6 / (double)rt->info.wall_clock_hz;
+  b->start_us.assign(b->count, -1.0);
+  b->end_us.assign(b->count, -1.0);
+  bool timed = true;
   b->reached.assign(b->count, 0);
   int64_t total = 0;
   for (int k = 0; k < b->count; k++) {
     b->reached[b->jobs[k].index] = (int64_t)per_job[k];
     b->levels[b->jobs[k].index] = iters[k];
+    if (jt[2 * k + 1] >= jt[2 * k] && jt[2 * k] != ~0ull) {
+      b->start_us[b->jobs[k].index] = (double)(jt[2 * k] - t0) * us_per_tick;
+      b->end_us[b->jobs[k].index] = (double)(jt[2 * k + 1] - t0) * us_per_tick;
+    } else {
+      timed = false;
+    }
     total += (int64_t)per_job[k];
+  }
+  if (timed) {
+    b->dur_us.resize(b->count);
+    for (int i = 0; i < b->count; i++) b->dur_us[i] = b->end_us[i] - b->start_us[i];
   }
   if (st) st->edges_scanned = total;
   return WGM_OK;
@@ -746,10 +887,12 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
 
 }  // namespace
 
+static int batch_lds(int wg) { return batch_lds_bytes(wg); }
+
 // Extension hooks used by runtime.hip
 bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
   if (kernel == WGM_K_BATCHED) {
-    *out = KernelInfo{(const void*)&k_batched<0>, wgm_frontier_lds};
+    *out = KernelInfo{(const void*)&k_batched<0>, batch_lds};
     return true;
   }
   return wgm_sched_kernel_info(kernel, out);
@@ -757,7 +900,7 @@ bool wgm_extra_kernel_info(int kernel, KernelInfo* out) {
 
 int wgm_discover_extra(wgm_runtime* rt, int kernel, int grid, int wg_size) {
   if (kernel == WGM_K_BATCHED) {
-    hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(wg_size), frontier_lds_bytes(wg_size), rt->stream, rt->ctl,
+    hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(wg_size), batch_lds_bytes(wg_size), rt->stream, rt->ctl,
                        BatchDev{});
     return WGM_OK;
   }
@@ -824,6 +967,7 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->tbar, sizeof(int32_t) * 2 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->ticket, sizeof(int32_t) * kLine);
   if (e == hipSuccess) e = alloc((void**)&b->iters, sizeof(int32_t) * count);
+  if (e == hipSuccess) e = alloc((void**)&b->jt, sizeof(unsigned long long) * 2 * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->d_reached, sizeof(unsigned long long) * count);
   if (e == hipSuccess) e = alloc((void**)&b->far, sizeof(int32_t) * b->total_f);
@@ -865,7 +1009,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->qa, b->qr,
-                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt};
+                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->jt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -888,7 +1032,11 @@ int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in,
   const bool same = b->exec && std::memcmp(&cfg, &b->graph_cfg, sizeof(cfg)) == 0 && grid == b->graph_grid;
   // The captured launch reads the device job table by pointer: make sure it
   // holds this config's table (another run may have rebuilt it meanwhile).
-  const bool table_ok = std::memcmp(&cfg, &b->jobs_cfg, sizeof(cfg)) == 0 && grid == b->jobs_grid;
+  // Rebuilt once more after the first timed run, in duration order (the
+  // captured launch reads the table by pointer; tickets and team sizes, its
+  // by-value arguments, depend only on the config).
+  const bool table_ok = std::memcmp(&cfg, &b->jobs_cfg, sizeof(cfg)) == 0 && grid == b->jobs_grid &&
+                        (b->jobs_by_time || b->dur_us.empty());
   if (!same || !cfg.use_graph || !table_ok) {
     rc = build_jobs(b, cfg, grid);
     if (rc) return rc;
@@ -964,10 +1112,11 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
     one.claims = d.claims + (size_t)k * 3 * kLine;
     one.tbar = d.tbar + (size_t)k * 2 * kLine;
     one.iters = d.iters + k;
+    one.jt = d.jt + 2 * k;
     rc = wgm_enqueue_reset_ctl(rt, rt->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, rt->stream, one);
-    hipLaunchKernelGGL(k_batched<1>, dim3(b->jobs[k].team), dim3(cfg.wg_size), frontier_lds_bytes(cfg.wg_size),
+    hipLaunchKernelGGL(k_batched<1>, dim3(b->jobs[k].team), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size),
                        rt->stream, rt->ctl, one);
   }
   HIPCHK(rt, hipGetLastError());
@@ -982,6 +1131,14 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
   if (rc) return rc;
   local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
   if (st) *st = local;
+  return WGM_OK;
+}
+
+int wgm_batch_job_times(const wgm_batch* b, double* start_us, double* end_us) {
+  if (!b || !start_us || !end_us) return fail(b ? b->rt : nullptr, WGM_ERR_INVALID, "bad job-times arguments");
+  if (b->start_us.empty()) return fail(b->rt, WGM_ERR_INVALID, "no run yet");
+  std::copy(b->start_us.begin(), b->start_us.end(), start_us);
+  std::copy(b->end_us.begin(), b->end_us.end(), end_us);
   return WGM_OK;
 }
 
@@ -1041,6 +1198,22 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
   for (int dl : deltas) {
     wgm_batch_config c = best;
     c.delta = dl;
+    for (int r = 0; r < reps; r++) {
+      wgm_batch_stats s{};
+      int rc = wgm_batch_run(rt, b, &c, &s);
+      if (rc) return rc;
+      if (s.wall_ms < best_ms) {
+        best_ms = s.wall_ms;
+        best = c;
+        bst = s;
+      }
+    }
+  }
+  // and the SSSP team granularity at the best shape and band
+  const int sssp_te[] = {1 << 16, 1 << 17, 1 << 18, 1 << 19};
+  for (int te : sssp_te) {
+    wgm_batch_config c = best;
+    c.sssp_team_edges = te;
     for (int r = 0; r < reps; r++) {
       wgm_batch_stats s{};
       int rc = wgm_batch_run(rt, b, &c, &s);

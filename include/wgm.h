@@ -505,7 +505,8 @@ typedef struct wgm_batch_config {
   int32_t max_team;    /* largest team */
   int32_t use_graph;   /* 1: capture the launch in a hipGraph and replay it */
   int32_t delta;       /* SSSP near-far band width (0: plain frontier Bellman-Ford) */
-  int32_t reserved;
+  int32_t sssp_team_edges; /* team granularity of SSSP queries (0: team_edges); an SSSP edge
+                              costs two to three BFS edges, so its teams can be larger */
 } wgm_batch_config;
 
 typedef struct wgm_batch_stats {
@@ -532,9 +533,14 @@ int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wg
 int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
 /* Result of query `index` of the last run (any output may be NULL). */
 int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* iterations, int64_t* edges_scanned);
+/* Per-query start and end of the last run (wgm_batch_run or _relaunch), in
+ * microseconds of the device wall clock from the first query's start; each
+ * array holds `count` values in caller order (-1: the query did not run). */
+int wgm_batch_job_times(const wgm_batch* b, double* start_us, double* end_us);
 /* Grid search over wg_size {64..1024}, wgs_per_cu {1,2,4,8}, team_edges
  * {2^14..2^20} and chunk {32..4096} (band 1024), then the SSSP near-far band
- * {0, 512..4096} at the best shape; returns the fastest configuration. */
+ * {0, 512..4096} and the SSSP team granularity {2^16..2^19} at the best
+ * shape; returns the fastest configuration. */
 int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best,
                        wgm_batch_stats* best_stats);
 
