@@ -236,6 +236,7 @@ __device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
 // into the conditional block that produced it, where it would wait for the
 // load or atomic before the next slot's is issued.
 __device__ __forceinline__ void opaque(uint32_t& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 
 // Frontier queues of the low-degree step: entry i of queue p (0 or 1) in
 // LDS below cap, in the HBM queue at index i above.
@@ -318,6 +319,15 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
       int* out_cnt = &s_cnt[(level + 1) % 3];
       const uint32_t next = (uint32_t)level + 1u;
       for (int a = 0; a < in_n; a += per * U) {
+        // slot x holds entries of this wave only while a + x per + its first
+        // slot < in_n: a scalar test, so a wave runs no code for empty slots
+        // (most levels fill one slot, and only some waves of it) -- issue
+        // time, not memory, bounds a narrow level
+        const int wslot = __builtin_amdgcn_readfirstlane((int)threadIdx.x & ~63) / K;
+        if (a + wslot >= in_n) continue;
+        bool lv[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) lv[x] = a + x * per + wslot < in_n;
         int v[U], e[U];
         uint32_t du[U];
         bool ok[U];
@@ -327,7 +337,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           uint32_t ri = 0;
           v[x] = 0;
           du[x] = 0;
-          if (i < in_n) lq.load(p, i, v[x], ri, du[x], SSSP);
+          if (lv[x] && i < in_n) lq.load(p, i, v[x], ri, du[x], SSSP);
           ok[x] = (int)(ri & 7u) > k;
           e[x] = (int)(ri >> 3) + k;
         }
@@ -336,6 +346,10 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         uint32_t w[U], dv[U];
 #pragma unroll
         for (int x = 0; x < U; x++) {
+          u[x] = 0;
+          w[x] = 0u;
+          dv[x] = 0u;
+          if (!lv[x]) continue;
           u[x] = ok[x] ? g.col[e[x]] : 0;
           w[x] = SSSP && ok[x] ? (uint32_t)g.wt[e[x]] : 0u;
           dv[x] = SSSP && ok[x] ? s.dist[v[x]] : 0u;
@@ -350,10 +364,14 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           alt[x] = SSSP ? du[x] + w[x] : next;
           // the row loads are unconditional (a dead slot reads row 0, which
           // every graph has): only the atomic needs the slot's condition
-          const int uu = live ? u[x] : 0;
+          r0[x] = 0u;
+          r1[x] = 0u;
+          old[x] = 0u;  // not live: never an improvement, never kInf
+          if (!lv[x]) continue;
+          int uu = live ? u[x] : 0;
+          opaque(uu);  // one unconditional load pair, not one per arm of live
           r0[x] = (uint32_t)g.row_ptr[uu];
           r1[x] = (uint32_t)g.row_ptr[uu + 1];
-          old[x] = 0u;  // not live: never an improvement, never kInf
           if (live) {
             // BFS: every distance set so far is <= level + 1 = alt, so the
             // minimum changes only kInf and old == kInf marks the discovery
@@ -378,6 +396,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         }
 #pragma unroll
         for (int x = 0; x < U; x++) {
+          if (!lv[x]) continue;
           const unsigned long long mask = __ballot(want[x]);
           if (mask) {
             const int leader = __ffsll((long long)mask) - 1;
