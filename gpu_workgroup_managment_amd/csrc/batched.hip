@@ -83,6 +83,8 @@ struct BatchDev {
   int* fcnt;       // 3 lines per job: far counts and band minimum
   uint32_t delta;  // near-far band width (0: off)
   unsigned long long* jt;  // per job: first member's start, last member's end (wall clock)
+  unsigned long long* reached;  // per job: out-degrees of the reached vertices, summed after the traversal
+  unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
 };
 
 // Wave-aggregated push of `item` (and its distance tag into qa, when given)
@@ -677,19 +679,46 @@ __global__ void k_batch_reset(BatchDev b) {
   for (int i = t; i < b.njobs * 2; i += gridDim.x * blockDim.x) {
     b.tbar[i * kLine] = 0;
     b.jt[i] = (i & 1) ? 0ull : ~0ull;
+    if (i < b.njobs) b.reached[i] = 0ull;
   }
   if (t == 0) *b.ticket = 0;
 }
 
-// Sum of out-degrees of reached vertices per job: one workgroup per job.
-__global__ void k_batch_reached(BatchDev b, unsigned long long* per_job) {
-  for (int j = blockIdx.x; j < b.njobs; j += gridDim.x) {
-    const BatchJob jb = b.jobs[j];
-    unsigned long long local = 0;
-    for (int v = threadIdx.x; v < jb.n; v += blockDim.x)
-      if (b.dist[jb.v_off + v] != kInf) local += (unsigned long long)(b.row[jb.row_off + v + 1] - b.row[jb.row_off + v]);
-    for (int d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(&per_job[j], local);
+// Sum of out-degrees of reached vertices per job, after the traversal:
+// job0 + blockIdx.y is the job, blockIdx.x a chunk of kReachChunk of its
+// vertices (the blocks past a small job's end leave at once), one atomic per
+// block.
+constexpr int kReachChunk = 4096;
+constexpr int kMaxGridY = 65535;
+__global__ void __launch_bounds__(256) k_batch_reached(BatchDev b, int job0) {
+  const int j = job0 + (int)blockIdx.y;
+  const BatchJob& jb = b.jobs[j];
+  const int v0 = (int)blockIdx.x * kReachChunk;
+  if (v0 >= jb.n) return;
+  const int v1 = min(v0 + kReachChunk, jb.n);
+  const uint32_t* dist = b.dist + jb.v_off;
+  const int32_t* row = b.row + jb.row_off;
+  unsigned long long local = 0;
+  for (int v = v0 + (int)threadIdx.x; v < v1; v += blockDim.x)
+    if (dist[v] != kInf) local += (unsigned long long)(row[v + 1] - row[v]);
+  for (int d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
+  __shared__ unsigned long long s_sum[4];
+  if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+    if (t) atomicAdd(&b.reached[j], t);
+  }
+}
+
+// The control cells the host reads after a run, next to the per-job results
+// so that one copy brings everything back.
+__global__ void k_batch_snap(Ctl c, BatchDev b) {
+  if (threadIdx.x == 0) {
+    b.snap[0] = c.clock[0];
+    b.snap[1] = c.clock[1];
+    b.snap[2] = (unsigned long long)(long long)*c.count;
+    b.snap[3] = (unsigned long long)(long long)c.lines[kLine];
   }
 }
 
@@ -709,10 +738,11 @@ struct wgm_batch {
   int32_t* claims = nullptr;
   BatchJob* d_jobs_one = nullptr;  // relaunch baseline: the same jobs, tickets from 0
   int chunk = 0;
-  int32_t *ticket = nullptr, *iters = nullptr;
-  unsigned long long* jt = nullptr;     // per job: start, end (wall clock)
+  int32_t *ticket = nullptr, *iters = nullptr;  // iters points into res
+  unsigned long long* jt = nullptr;     // per job: start, end (wall clock); points into res
   std::vector<double> start_us, end_us;  // per caller index, after a run
   std::vector<double> dur_us;            // per caller index: last run's duration (empty before one)
+  int short_team = 0;                    // discovered workgroups when a team could not form (0: none)
   bool jobs_by_time = false;             // job table ordered by measured durations
   uint32_t *dist = nullptr, *qa = nullptr, *qr = nullptr;
   std::vector<int32_t> qcap;      // entries per queue, per graph
@@ -723,7 +753,10 @@ struct wgm_batch {
   std::vector<int64_t> f_off;
   uint32_t delta = 0;             // near-far band of the current job table
   BatchJob* d_jobs = nullptr;
-  unsigned long long* d_reached = nullptr;
+  // per-job results and control snapshot in one block (one copy per run):
+  // reached[count], jt[2 count], snap[4], then iters (int32)[count]
+  unsigned long long* res = nullptr;
+  size_t res_bytes = 0;
   std::vector<BatchJob> jobs;           // current job table (LPT order)
   wgm_batch_config jobs_cfg{};          // config and grid the device job table was built for
   int jobs_grid = -1;
@@ -773,6 +806,8 @@ BatchDev dev_view(wgm_batch* b) {
   d.ticket = b->ticket;
   d.iters = b->iters;
   d.jt = b->jt;
+  d.reached = b->res;
+  d.snap = b->res + 3 * (size_t)b->count;
   d.far = b->far;
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
@@ -852,6 +887,20 @@ int batch_grid(wgm_runtime* rt, const wgm_batch_config& cfg, int* grid) {
 }
 
 // Enqueue one persistent batched launch (reset kernels + the kernel).
+// After the traversal: the reached-degree sums and the control snapshot
+// (part of the captured launch, so a run ends with one copy to the host).
+int enqueue_collect(wgm_batch* b, hipStream_t st) {
+  wgm_runtime* rt = b->rt;
+  const BatchDev d = dev_view(b);
+  const int max_n = *std::max_element(b->n.begin(), b->n.end());
+  for (int j0 = 0; j0 < b->count; j0 += kMaxGridY)
+    hipLaunchKernelGGL(k_batch_reached, dim3((max_n + kReachChunk - 1) / kReachChunk, std::min(kMaxGridY, b->count - j0)),
+                       dim3(256), 0, st, d, j0);
+  hipLaunchKernelGGL(k_batch_snap, dim3(1), dim3(64), 0, st, rt->ctl, d);
+  WGM_HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
 int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStream_t st) {
   wgm_runtime* rt = b->rt;
   BatchDev d = dev_view(b);
@@ -859,26 +908,33 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
-  WGM_HIPCHK(rt, hipGetLastError());
-  return WGM_OK;
+  return enqueue_collect(b, st);
 }
 
+// Ends a run: waits for the stream (event ev1 after the work), copies the
+// results block once and fills the per-query results and the stats.
 int collect_results(wgm_batch* b, wgm_batch_stats* st) {
   wgm_runtime* rt = b->rt;
-  BatchDev d = dev_view(b);
-  WGM_HIPCHK(rt, hipMemsetAsync(b->d_reached, 0, sizeof(unsigned long long) * b->count, rt->stream));
-  hipLaunchKernelGGL(k_batch_reached, dim3(std::min(b->count, 1024)), dim3(256), 0, rt->stream, d, b->d_reached);
-  WGM_HIPCHK(rt, hipGetLastError());
-  std::vector<unsigned long long> per_job(b->count), jt(2 * (size_t)b->count);
-  std::vector<int> iters(b->count);
-  WGM_HIPCHK(rt, hipMemcpyAsync(per_job.data(), b->d_reached, sizeof(unsigned long long) * b->count,
-                                hipMemcpyDeviceToHost, rt->stream));
-  WGM_HIPCHK(rt, hipMemcpyAsync(iters.data(), b->iters, sizeof(int) * b->count, hipMemcpyDeviceToHost, rt->stream));
-  WGM_HIPCHK(rt, hipMemcpyAsync(jt.data(), b->jt, sizeof(unsigned long long) * jt.size(), hipMemcpyDeviceToHost,
-                                rt->stream));
+  WGM_HIPCHK(rt, hipEventRecord(rt->ev1, rt->stream));
+  std::vector<unsigned char> raw(b->res_bytes);
+  WGM_HIPCHK(rt, hipMemcpyAsync(raw.data(), b->res, b->res_bytes, hipMemcpyDeviceToHost, rt->stream));
   WGM_HIPCHK(rt, hipStreamSynchronize(rt->stream));
+  float ms = 0.f;
+  WGM_HIPCHK(rt, hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
+  const size_t nj = (size_t)b->count;
+  const unsigned long long* per_job = reinterpret_cast<const unsigned long long*>(raw.data());
+  const unsigned long long* jt = per_job + nj;
+  const unsigned long long* snap = per_job + 3 * nj;
+  const int32_t* iters = reinterpret_cast<const int32_t*>(per_job + 3 * nj + 4);
+  if (st) {
+    st->wall_ms = ms;
+    const unsigned long long c0 = snap[0], c1 = snap[1];
+    st->device_ms = (c1 >= c0 && c0) ? (double)(c1 - c0) * 1e3 / (double)rt->info.wall_clock_hz : 0.0;
+    st->participating = (int32_t)(long long)snap[2];
+  }
+  b->short_team = (int)(long long)snap[3];
   unsigned long long t0 = ~0ull;
-  for (int k = 0; k < b->count; k++) t0 = std::min(t0, jt[2 * k]);
+  for (size_t k = 0; k < nj; k++) t0 = std::min(t0, jt[2 * k]);
   const double us_per_tick = 1e6 / (double)rt->info.wall_clock_hz;
   b->start_us.assign(b->count, -1.0);
   b->end_us.assign(b->count, -1.0);
@@ -985,10 +1041,13 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs_one, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->tbar, sizeof(int32_t) * 2 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->ticket, sizeof(int32_t) * kLine);
-  if (e == hipSuccess) e = alloc((void**)&b->iters, sizeof(int32_t) * count);
-  if (e == hipSuccess) e = alloc((void**)&b->jt, sizeof(unsigned long long) * 2 * count);
+  b->res_bytes = sizeof(unsigned long long) * (3 * (size_t)count + 4) + sizeof(int32_t) * (size_t)count;
+  if (e == hipSuccess) e = alloc((void**)&b->res, b->res_bytes);
+  if (e == hipSuccess) {
+    b->jt = b->res + count;
+    b->iters = reinterpret_cast<int32_t*>(b->res + 3 * (size_t)count + 4);
+  }
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
-  if (e == hipSuccess) e = alloc((void**)&b->d_reached, sizeof(unsigned long long) * count);
   if (e == hipSuccess) e = alloc((void**)&b->far, sizeof(int32_t) * b->total_f);
   if (e == hipSuccess) e = alloc((void**)&b->in_far, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = alloc((void**)&b->fcnt, sizeof(int32_t) * 3 * kLine * count);
@@ -1027,8 +1086,8 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (!b) return WGM_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->iters, b->dist, b->qa, b->qr,
-                  b->d_jobs, b->d_reached, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->jt};
+  void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->res, b->dist, b->qa, b->qr,
+                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -1083,22 +1142,14 @@ int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in,
     rc = enqueue_batched(b, cfg, grid, rt->stream);
     if (rc) return rc;
   }
-  wgm_run_stats rs{};
-  rc = wgm_finish_launch(rt, &rs);
-  if (rc) return rc;
-  int short_team = 0;
-  HIPCHK(rt, hipMemcpy(&short_team, rt->ctl.lines + kLine, sizeof(int), hipMemcpyDeviceToHost));
-  if (short_team)
-    return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "only %d workgroups discovered for teams of up to %d", short_team,
-                dev_view(b).max_team);
   wgm_batch_stats local{};
-  local.wall_ms = rs.kernel_ms;
-  local.device_ms = rs.device_ms;
-  local.participating = rs.participating;
   local.launches = 1;
   local.grid = grid;
   rc = collect_results(b, &local);
   if (rc) return rc;
+  if (b->short_team)
+    return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "only %d workgroups discovered for teams of up to %d", b->short_team,
+                dev_view(b).max_team);
   local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
   if (st) *st = local;
   return WGM_OK;
@@ -1132,6 +1183,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
     one.tbar = d.tbar + (size_t)k * 2 * kLine;
     one.iters = d.iters + k;
     one.jt = d.jt + 2 * k;
+    one.reached = d.reached + k;
     rc = wgm_enqueue_reset_ctl(rt, rt->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, rt->stream, one);
@@ -1139,15 +1191,15 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
                        rt->stream, rt->ctl, one);
   }
   HIPCHK(rt, hipGetLastError());
-  wgm_run_stats rs{};
-  rc = wgm_finish_launch(rt, &rs);
+  rc = enqueue_collect(b, rt->stream);
   if (rc) return rc;
   wgm_batch_stats local{};
-  local.wall_ms = rs.kernel_ms;
   local.launches = b->count;
   local.grid = grid;
   rc = collect_results(b, &local);
   if (rc) return rc;
+  local.device_ms = 0.0;  // per-launch clocks: only the wall time spans the whole baseline
+  local.participating = 0;
   local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
   if (st) *st = local;
   return WGM_OK;
