@@ -242,13 +242,18 @@ __device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 
 // Frontier queues of the low-degree step: entry i of queue p (0 or 1) in
 // LDS below cap, in the HBM queue at index i above.
+// LDS-typed, so that the two paths of load/store stay ds_* and global_*
+// instructions: through a generic pointer the compiler merges them into one
+// flat access, which then waits on both the LDS and the memory counters.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 struct LowQueues {
-  uint32_t* lds;  // [queue][vertex, row info, tag][cap]
+  lds_u32* lds;  // [queue][vertex, row info, tag][cap]
   int cap;
   const FrontierDev* s;
   __device__ __forceinline__ void load(int p, int i, int& v, uint32_t& ri, uint32_t& du, bool tag) const {
     if (i < cap) {
-      const uint32_t* b = lds + p * 3 * cap;
+      const lds_u32* b = lds + p * 3 * cap;
       v = (int)b[i];
       ri = b[cap + i];
       du = tag ? b[2 * cap + i] : 0u;
@@ -260,7 +265,7 @@ struct LowQueues {
   }
   __device__ __forceinline__ void store(int p, int i, int v, uint32_t ri, uint32_t du, bool tag) const {
     if (i < cap) {
-      uint32_t* b = lds + p * 3 * cap;
+      lds_u32* b = lds + p * 3 * cap;
       b[i] = (uint32_t)v;
       b[cap + i] = ri;
       if (tag) b[2 * cap + i] = du;
@@ -289,7 +294,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
                         int* iters_out) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
-  const LowQueues lq{reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4,
+  const LowQueues lq{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
                      low_lds_cap((int)blockDim.x), &s};
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
