@@ -410,6 +410,8 @@ def main():
             "vertices": int(sum(g.n for g in graphs)),
             "edges": int(sum(g.m for g in graphs)),
             "edges_scanned_per_step": int(edges),
+            "edges_relaxed_per_step": int(st["edges_relaxed"]),
+            "relaxations_per_scanned_edge": round(st["edges_relaxed"] / max(edges, 1), 4),
             "launch": cfg_d,
             "participating_wgs": st["participating"],
         },

@@ -145,6 +145,7 @@ class BatchStats(ctypes.Structure):
         ("launches", ctypes.c_int32),
         ("grid", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("edges_relaxed", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -84,6 +84,7 @@ struct BatchDev {
   uint32_t delta;  // near-far band width (0: off)
   unsigned long long* jt;  // per job: first member's start, last member's end (wall clock)
   unsigned long long* reached;  // per job: out-degrees of the reached vertices, summed after the traversal
+  unsigned long long* relaxed;  // per job: edges relaxed from live frontier entries (work)
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
 };
 
@@ -316,6 +317,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   const unsigned long long below = (1ull << lane) - 1ull;
   int level = 0, fi = 0;
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
+  uint32_t nrel = 0;  // edges this thread relaxed (live entries' edges)
   while (true) {
     for (;; level++) {
       const int in_n = __builtin_amdgcn_readfirstlane(s_cnt[level % 3]);
@@ -368,6 +370,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
 #pragma unroll
         for (int x = 0; x < U; x++) {
           const bool live = ok[x] && (!SSSP || dv[x] == du[x]);
+          nrel += live ? 1u : 0u;
           alt[x] = SSSP ? du[x] + w[x] : next;
           // the row loads are unconditional (a dead slot reads row 0, which
           // every graph has): only the atomic needs the slot's condition
@@ -493,6 +496,8 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
     fi = 1 - fi;
     __syncthreads();
   }
+  for (int d = 32; d > 0; d >>= 1) nrel += __shfl_down(nrel, d, 64);
+  if ((threadIdx.x & 63) == 0 && nrel && s.relaxed) atomicAdd(s.relaxed, (unsigned long long)nrel);
   if (threadIdx.x == 0) *iters_out = level;
 }
 
@@ -636,6 +641,7 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     s.counts = b.counts + (size_t)j * 3 * kLine;
     s.claims = b.claims + (size_t)j * 3 * kLine;
     s.chunk = b.chunk;
+    s.relaxed = b.relaxed + j;
     const bool sssp = jb.query == WGM_QUERY_SSSP;
     NearFar nf{};
     nf.delta = sssp && b.far ? b.delta : 0u;
@@ -684,7 +690,10 @@ __global__ void k_batch_reset(BatchDev b) {
   for (int i = t; i < b.njobs * 2; i += gridDim.x * blockDim.x) {
     b.tbar[i * kLine] = 0;
     b.jt[i] = (i & 1) ? 0ull : ~0ull;
-    if (i < b.njobs) b.reached[i] = 0ull;
+    if (i < b.njobs) {
+      b.reached[i] = 0ull;
+      b.relaxed[i] = 0ull;
+    }
   }
   if (t == 0) *b.ticket = 0;
 }
@@ -759,7 +768,7 @@ struct wgm_batch {
   uint32_t delta = 0;             // near-far band of the current job table
   BatchJob* d_jobs = nullptr;
   // per-job results and control snapshot in one block (one copy per run):
-  // reached[count], jt[2 count], snap[4], then iters (int32)[count]
+  // reached[count], jt[2 count], relaxed[count], snap[4], then iters (int32)[count]
   unsigned long long* res = nullptr;
   size_t res_bytes = 0;
   std::vector<BatchJob> jobs;           // current job table (LPT order)
@@ -812,7 +821,8 @@ BatchDev dev_view(wgm_batch* b) {
   d.iters = b->iters;
   d.jt = b->jt;
   d.reached = b->res;
-  d.snap = b->res + 3 * (size_t)b->count;
+  d.relaxed = b->res + 3 * (size_t)b->count;
+  d.snap = b->res + 4 * (size_t)b->count;
   d.far = b->far;
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
@@ -929,8 +939,9 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
   const size_t nj = (size_t)b->count;
   const unsigned long long* per_job = reinterpret_cast<const unsigned long long*>(raw.data());
   const unsigned long long* jt = per_job + nj;
-  const unsigned long long* snap = per_job + 3 * nj;
-  const int32_t* iters = reinterpret_cast<const int32_t*>(per_job + 3 * nj + 4);
+  const unsigned long long* relaxed = per_job + 3 * nj;
+  const unsigned long long* snap = per_job + 4 * nj;
+  const int32_t* iters = reinterpret_cast<const int32_t*>(per_job + 4 * nj + 4);
   if (st) {
     st->wall_ms = ms;
     const unsigned long long c0 = snap[0], c1 = snap[1];
@@ -945,7 +956,7 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
   b->end_us.assign(b->count, -1.0);
   bool timed = true;
   b->reached.assign(b->count, 0);
-  int64_t total = 0;
+  int64_t total = 0, relax_total = 0;
   for (int k = 0; k < b->count; k++) {
     b->reached[b->jobs[k].index] = (int64_t)per_job[k];
     b->levels[b->jobs[k].index] = iters[k];
@@ -956,12 +967,16 @@ int collect_results(wgm_batch* b, wgm_batch_stats* st) {
       timed = false;
     }
     total += (int64_t)per_job[k];
+    relax_total += (int64_t)relaxed[k];
   }
   if (timed) {
     b->dur_us.resize(b->count);
     for (int i = 0; i < b->count; i++) b->dur_us[i] = b->end_us[i] - b->start_us[i];
   }
-  if (st) st->edges_scanned = total;
+  if (st) {
+    st->edges_scanned = total;
+    st->edges_relaxed = relax_total;
+  }
   return WGM_OK;
 }
 
@@ -1046,11 +1061,11 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs_one, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->tbar, sizeof(int32_t) * 2 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->ticket, sizeof(int32_t) * kLine);
-  b->res_bytes = sizeof(unsigned long long) * (3 * (size_t)count + 4) + sizeof(int32_t) * (size_t)count;
+  b->res_bytes = sizeof(unsigned long long) * (4 * (size_t)count + 4) + sizeof(int32_t) * (size_t)count;
   if (e == hipSuccess) e = alloc((void**)&b->res, b->res_bytes);
   if (e == hipSuccess) {
     b->jt = b->res + count;
-    b->iters = reinterpret_cast<int32_t*>(b->res + 3 * (size_t)count + 4);
+    b->iters = reinterpret_cast<int32_t*>(b->res + 4 * (size_t)count + 4);
   }
   if (e == hipSuccess) e = alloc((void**)&b->d_jobs, sizeof(BatchJob) * count);
   if (e == hipSuccess) e = alloc((void**)&b->far, sizeof(int32_t) * b->total_f);
@@ -1189,6 +1204,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
     one.iters = d.iters + k;
     one.jt = d.jt + 2 * k;
     one.reached = d.reached + k;
+    one.relaxed = d.relaxed + k;
     rc = wgm_enqueue_reset_ctl(rt, rt->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, rt->stream, one);

@@ -51,6 +51,9 @@ struct FrontierDev {
   // low-degree path, which loads it when it queues the entry)
   uint32_t* qr0;
   uint32_t* qr1;
+  // when set: edges relaxed (out-degrees of the live entries processed) are
+  // added here, one atomic per frontier batch -- the traversal's work
+  unsigned long long* relaxed;
 };
 
 // Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17], claim[1]
@@ -104,6 +107,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   }
   int total;
   const int excl = wg_exclusive_scan(deg, s_wave, &total);
+  if (s.relaxed && tid == 0 && total > 0) atomicAdd(s.relaxed, (unsigned long long)total);
   s_end[tid] = excl + deg;
   s_beg[tid] = beg - excl;
   if (SSSP) s_du[tid] = du;

@@ -518,6 +518,7 @@ typedef struct wgm_batch_stats {
   int32_t launches;        /* kernel launches that served the queries */
   int32_t grid;            /* launch grid of the persistent kernel */
   int32_t reserved;
+  int64_t edges_relaxed;   /* edges relaxed from live frontier entries (work: BFS = edges_scanned, SSSP >= it) */
 } wgm_batch_stats;
 
 /* Upload `count` graphs (host CSR; weights needed for SSSP queries) into
