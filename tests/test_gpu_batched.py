@@ -67,3 +67,25 @@ def test_batched_wg_sizes(rt, small_batch, wg):
     b.run(wg_size=wg, wgs_per_cu=1, team_edges=1024, max_team=16)
     _check_all(b, graphs, refs)
     b.close()
+
+
+def test_batched_job_times_work_and_order(rt, small_batch):
+    """Per-query device stamps, relaxation counts and the duration-ordered
+    job table: BFS relaxes each reached vertex's edges exactly once, SSSP at
+    least once; after a timed run the table is rebuilt longest-first and the
+    results stay exact, also for a separate SSSP team granularity."""
+    graphs, queries, refs = small_batch
+    bfs = [i for i, q in enumerate(queries) if q == L.QUERY_BFS]
+    b = B.Batch(rt, [graphs[i] for i in bfs], queries[bfs])
+    st = b.run(wg_size=256, wgs_per_cu=2, team_edges=512, max_team=8)
+    assert st["edges_relaxed"] == st["edges_scanned"]
+    b.close()
+    b = B.Batch(rt, graphs, queries)
+    for rep in range(3):  # first run by edge count, then by measured duration
+        st = b.run(wg_size=256, wgs_per_cu=2, team_edges=512, max_team=8, delta=64, sssp_team_edges=256)
+        _check_all(b, graphs, refs)
+        assert st["edges_relaxed"] >= st["edges_scanned"]
+        t0, t1 = b.job_times()
+        assert t0.shape == (len(graphs),) and np.all(t0 >= 0) and np.all(t1 >= t0)
+        assert t0.min() == 0.0
+    b.close()
