@@ -700,22 +700,34 @@ __global__ void k_batch_reset(BatchDev b) {
 
 // Sum of out-degrees of reached vertices per job, after the traversal:
 // job0 + blockIdx.y is the job, blockIdx.x a chunk of kReachChunk of its
-// vertices (the blocks past a small job's end leave at once), one atomic per
-// block.
-constexpr int kReachChunk = 4096;
+// vertices (the blocks past a small job's end leave at once). Each thread
+// issues all kReachPer of its loads before summing (one round of memory
+// latency per block), then one atomic per block.
+constexpr int kReachPer = 8;
+constexpr int kReachChunk = 256 * kReachPer;
 constexpr int kMaxGridY = 65535;
 __global__ void __launch_bounds__(256) k_batch_reached(BatchDev b, int job0) {
   const int j = job0 + (int)blockIdx.y;
-  const BatchJob& jb = b.jobs[j];
+  const int n = b.jobs[j].n;
   const int v0 = (int)blockIdx.x * kReachChunk;
-  if (v0 >= jb.n) return;
-  const int v1 = min(v0 + kReachChunk, jb.n);
-  const uint32_t* dist = b.dist + jb.v_off;
-  const int32_t* row = b.row + jb.row_off;
+  if (v0 >= n) return;
+  const uint32_t* dist = b.dist + b.jobs[j].v_off;
+  const int32_t* row = b.row + b.jobs[j].row_off;
+  uint32_t d[kReachPer];
+  int r0[kReachPer], r1[kReachPer];
+#pragma unroll
+  for (int k = 0; k < kReachPer; k++) {
+    const int v = v0 + k * 256 + (int)threadIdx.x;
+    const bool in = v < n;
+    d[k] = in ? dist[v] : kInf;
+    r0[k] = in ? row[v] : 0;
+    r1[k] = in ? row[v + 1] : 0;
+  }
   unsigned long long local = 0;
-  for (int v = v0 + (int)threadIdx.x; v < v1; v += blockDim.x)
-    if (dist[v] != kInf) local += (unsigned long long)(row[v + 1] - row[v]);
-  for (int d = 32; d > 0; d >>= 1) local += __shfl_down(local, d, 64);
+#pragma unroll
+  for (int k = 0; k < kReachPer; k++)
+    if (d[k] != kInf) local += (unsigned long long)(r1[k] - r0[k]);
+  for (int s = 32; s > 0; s >>= 1) local += __shfl_down(local, s, 64);
   __shared__ unsigned long long s_sum[4];
   if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = local;
   __syncthreads();
