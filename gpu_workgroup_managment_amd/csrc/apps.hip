@@ -10,7 +10,9 @@
 // resident and only the per-run state is re-initialised on the device.
 
 #include "apps_internal.hpp"
+#include "team_frontier.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 using namespace wgm;
@@ -22,19 +24,22 @@ namespace wgm {
 // discovery; host handshake; barrier; the application loop with global
 // barriers (here the XCD-aware two-level barrier); retire.
 
+// Band width of the standalone SSSP's near-far bands (the batched launch's
+// tuned width for weights in [1, 255]).
+constexpr uint32_t kStandaloneDelta = 1024;
+
+// BFS / SSSP over the whole grid as one team (run_team): state shared through
+// agent-scope atomics, so the per-level two-level barrier skips its L2
+// write-back and invalidation (which would refetch the graph every level);
+// SSSP queue entries carry distance tags and run in near-far bands.
 template <bool SSSP>
-__global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDev s, int* iters_out) {
+__global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDev s, NearFar nf, int src,
+                                                   int* iters_out) {
   extern __shared__ int lds[];
   Team t;
   if (!standalone_begin(c, t)) return;
-  int level = 0;
-  while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-    frontier_level<SSSP>(g, s, level, t.id, t.n, lds);
-    xcd_barrier(c, t);
-    level++;
-  }
+  run_team<SSSP>(g, s, nf, src, t.id, t.n, lds, iters_out, [&]() { xcd_barrier<false>(c, t); });
   clock_end(c, t.id);
-  if (t.id == 0 && threadIdx.x == 0) *iters_out = level;
   standalone_retire(c);
 }
 
@@ -108,20 +113,6 @@ __global__ void k_reached_edges(GraphDev g, const uint32_t* dist, unsigned long 
 }
 
 
-__global__ void k_frontier_init(FrontierDev s, int n, int src) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int stride = gridDim.x * blockDim.x;
-  for (int v = t; v < n; v += stride) {
-    s.dist[v] = (v == src) ? 0u : kInf;
-    if (s.stamp) s.stamp[v] = 0u;
-  }
-  if (t == 0) {
-    s.q0[0] = src;
-    s.counts[0] = 1;
-    s.counts[kLine] = 0;
-    s.counts[2 * kLine] = 0;
-  }
-}
 
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
@@ -204,6 +195,57 @@ int wgm_graph_frontier_state(wgm_graph* g, bool sssp, FrontierDev* s) {
   return WGM_OK;
 }
 
+// State of the standalone team traversal: queues of max(n, m) + 1 entries
+// with distance tags for SSSP (a level queues at most one entry per
+// relaxation), n + 1 for BFS; claim counters; near-far queues and flags.
+static int team_frontier_state(wgm_graph* g, bool sssp, FrontierDev* s, NearFar* nf) {
+  wgm_runtime* rt = g->rt;
+  const int64_t cap = (sssp ? std::max<int64_t>(g->n, g->m) : (int64_t)g->n) + 1;
+  if (g->tcap < cap) {  // a BFS-sized pair of queues cannot serve SSSP
+    (void)hipFree(g->tq);
+    (void)hipFree(g->tqa);
+    g->tq = nullptr;
+    g->tqa = nullptr;
+    g->tcap = 0;
+  }
+  int rc = dev_alloc(rt, &g->dist, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->tq, 2 * (size_t)cap);
+  if (!rc && sssp) rc = dev_alloc(rt, &g->tqa, 2 * (size_t)std::max<int64_t>(cap, g->tcap));
+  if (!rc) rc = dev_alloc(rt, &g->counts, 3 * kLine);
+  if (!rc) rc = dev_alloc(rt, &g->claims, 3 * kLine);
+  if (!rc && sssp) rc = dev_alloc(rt, &g->far, 2 * ((size_t)g->n + 1));
+  if (!rc && sssp) rc = dev_alloc(rt, &g->in_far, (size_t)g->n);
+  if (!rc && sssp) rc = dev_alloc(rt, &g->fcnt, 3 * kLine);
+  const size_t hcap = (size_t)(g->m / kHeavyMin) + 2;
+  if (!rc) rc = dev_alloc(rt, &g->hv, hcap);
+  if (!rc) rc = dev_alloc(rt, &g->hdu, hcap);
+  if (!rc) rc = dev_alloc(rt, &g->hclaim, hcap);
+  if (!rc) rc = dev_alloc(rt, &g->hcnt, 3 * kLine);
+  if (rc) return rc;
+  if (g->tcap < cap) g->tcap = cap;
+  *s = FrontierDev{};
+  s->dist = g->dist;
+  s->q0 = g->tq;
+  s->q1 = g->tq + g->tcap;
+  s->qa0 = sssp ? g->tqa : nullptr;
+  s->qa1 = sssp ? g->tqa + g->tcap : nullptr;
+  s->counts = g->counts;
+  s->claims = g->claims;
+  s->hv = g->hv;
+  s->hdu = g->hdu;
+  s->hclaim = g->hclaim;
+  s->hcnt = g->hcnt;
+  *nf = NearFar{};
+  if (sssp) {
+    nf->q[0] = g->far;
+    nf->q[1] = g->far + g->n + 1;
+    nf->flag = g->in_far;
+    nf->cnt = g->fcnt;
+    nf->delta = kStandaloneDelta;
+  }
+  return WGM_OK;
+}
+
 int wgm_graph_reached_edges(wgm_graph* g, int64_t* out) {
   wgm_runtime* rt = g->rt;
   unsigned long long* d = nullptr;
@@ -237,11 +279,11 @@ int wgm_app_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
   switch (kernel) {
     case WGM_K_BFS:
       hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg_size), lds, rt->stream, rt->ctl, GraphDev{},
-                         FrontierDev{}, (int*)nullptr);
+                         FrontierDev{}, NearFar{}, 0, (int*)nullptr);
       return WGM_OK;
     case WGM_K_SSSP:
       hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg_size), lds, rt->stream, rt->ctl, GraphDev{},
-                         FrontierDev{}, (int*)nullptr);
+                         FrontierDev{}, NearFar{}, 0, (int*)nullptr);
       return WGM_OK;
     case WGM_K_COLOR:
       hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (int32_t*)nullptr,
@@ -296,7 +338,8 @@ int wgm_graph_destroy(wgm_graph* g) {
   if (!g) return WGM_OK;
   wgm_pannotia_release(g);
   void* ptrs[] = {g->row_ptr, g->col, g->wt, g->prio, g->dist, g->stamp, g->q0, g->q1, g->counts,
-                  g->i32a, g->i32b, g->f0, g->f1, g->f2, g->iters};
+                  g->i32a, g->i32b, g->f0, g->f1, g->f2, g->iters, g->tq, g->tqa, g->claims, g->far,
+                  g->in_far, g->fcnt, g->hv, g->hdu, g->hclaim, g->hcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete g;
@@ -318,18 +361,22 @@ static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_la
   if (src < 0 || src >= g->n) return fail(rt, WGM_ERR_INVALID, "source %d out of range", src);
   if (sssp && !g->wt) return fail(rt, WGM_ERR_INVALID, "SSSP needs edge weights");
   FrontierDev s{};
-  int rc = wgm_graph_frontier_state(g, sssp, &s);
+  NearFar nf{};
+  int rc = team_frontier_state(g, sssp, &s, &nf);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_frontier_init, dim3(256), dim3(256), 0, rt->stream, s, g->n, src);
   const int kernel = sssp ? WGM_K_SSSP : WGM_K_BFS;
   int grid = 0, wg = 0;
   rc = launch_prologue(rt, kernel, lc, &grid, &wg);
   if (rc) return rc;
+  s.chunk = wg;  // frontier entries a workgroup claims at a time
+  s.heavy = heavy_threshold(wg);
   const size_t lds = frontier_lds_bytes(wg);
   if (sssp)
-    hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, g->iters);
+    hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, nf,
+                       src, g->iters);
   else
-    hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, g->iters);
+    hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, nf,
+                       src, g->iters);
   wgm_run_stats local{};
   rc = launch_epilogue(rt, lc, g, &local);
   if (rc) return rc;

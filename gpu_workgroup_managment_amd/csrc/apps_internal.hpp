@@ -23,6 +23,20 @@ struct wgm_graph {
   float* f1 = nullptr;
   float* f2 = nullptr;
   int* iters = nullptr;
+  // standalone BFS / SSSP (one team of every participant): two queues of
+  // tcap entries with distance tags (SSSP), claim counters, near-far queues
+  // and flags
+  int32_t* tq = nullptr;
+  uint32_t* tqa = nullptr;
+  int64_t tcap = 0;
+  int* claims = nullptr;
+  int32_t* far = nullptr;
+  uint32_t* in_far = nullptr;
+  int* fcnt = nullptr;
+  int32_t* hv = nullptr;  // heavy lists: m / kHeavyMin + 2 entries
+  uint32_t* hdu = nullptr;
+  int32_t* hclaim = nullptr;
+  int* hcnt = nullptr;    // 3 lines
   // in-edge CSR and min-plus tasks (Pannotia SSSP / BC), built on first use
   int32_t* rrow = nullptr;
   int32_t* rcol = nullptr;

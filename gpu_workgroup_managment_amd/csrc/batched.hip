@@ -15,6 +15,7 @@
 // puller co-resident (discovery), no team waits forever.
 
 #include "apps_internal.hpp"
+#include "team_frontier.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -88,61 +89,6 @@ struct BatchDev {
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
 };
 
-// Wave-aggregated push of `item` (and its distance tag into qa, when given)
-// where `want`, one counter atomic per wave.
-template <bool COH>
-__device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* cnt, int scope,
-                                          uint32_t* qa = nullptr, uint32_t tag = 0) {
-  const unsigned long long mask = __ballot(want);
-  if (!mask) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) {
-    base = scope == __HIP_MEMORY_SCOPE_WORKGROUP
-               ? __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-               : __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  base = __shfl(base, leader, 64);
-  if (want) {
-    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-    if (COH) {
-      __hip_atomic_store(&q[pos], item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (qa) __hip_atomic_store(&qa[pos], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      q[pos] = item;
-      if (qa) qa[pos] = tag;
-    }
-  }
-}
-
-__device__ __forceinline__ int wave_min_int(int x) {
-  for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
-  return x;
-}
-
-// Counter barrier of a team of k workgroups (k small). Arrival is a fetch_add
-// on a monotonic counter; the arrival that completes epoch e publishes
-// generation e. Everything the team shares is accessed with agent-scope
-// atomics (frontier_batch<..., COH>), so the barrier only has to drain this
-// workgroup's outstanding accesses before arriving: no L2 write-back or
-// invalidation, which would also evict every other job's cached graph on the
-// XCD.
-__device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epoch) {
-  drain_vmem();
-  __syncthreads();
-  epoch++;
-  if (threadIdx.x == 0) {
-    const int old = add_relaxed(cnt, 1);
-    if (old == epoch * k - 1) {
-      st_relaxed(gen, epoch);
-    } else {
-      while (ld_relaxed(gen) < epoch) {
-      }
-    }
-  }
-  __syncthreads();
-}
 
 // A graph served by one workgroup: level counters live in LDS and the
 // workgroup's own barrier orders the levels; all atomics are workgroup scope.
@@ -235,11 +181,6 @@ __device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
   return ((uint32_t)rs << 3) | (uint32_t)(g.row_ptr[v + 1] - rs);
 }
 
-// An empty register barrier: the compiler may not fold a later use of v back
-// into the conditional block that produced it, where it would wait for the
-// load or atomic before the next slot's is issued.
-__device__ __forceinline__ void opaque(uint32_t& v) { asm volatile("" : "+v"(v)); }
-__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 
 // Frontier queues of the low-degree step: entry i of queue p (0 or 1) in
 // LDS below cap, in the HBM queue at index i above.
@@ -501,88 +442,6 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   if (threadIdx.x == 0) *iters_out = level;
 }
 
-// A graph served by a team of k > 1 workgroups (near-far as in run_single,
-// with the far counters and band minimum in device lines).
-template <bool SSSP>
-__device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int rank, int k,
-                         int* tcnt, int* tgen, int* lds, int* iters_out) {
-  const bool near_far = SSSP && nf.delta > 0;
-  int epoch = 0;
-  int lo, hi;
-  block_range(g.n, rank, k, &lo, &hi);
-  // initial state through the coherent path (other members' atomics see it)
-  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-    __hip_atomic_store(&s.dist[v], (v == src) ? 0u : kInf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (near_far) __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (rank == 0 && threadIdx.x == 0) {
-    __hip_atomic_store(&s.q0[0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (SSSP) __hip_atomic_store(&s.qa0[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    st_relaxed(&s.counts[0], 1);
-    st_relaxed(&s.counts[kLine], 0);
-    st_relaxed(&s.counts[2 * kLine], 0);
-    for (int l = 0; l < 3; l++) st_relaxed(&s.claims[l * kLine], 0);
-    if (near_far) {
-      st_relaxed(&nf.cnt[0], 0);
-      st_relaxed(&nf.cnt[kLine], 0);
-      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
-    }
-  }
-  team_barrier(tcnt, tgen, k, epoch);
-  const int bd = blockDim.x;
-  int level = 0, fi = 0;
-  uint32_t thr = near_far ? nf.delta : 0xffffffffu;
-  while (true) {
-    while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-      frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
-                                 nf.flag);
-      team_barrier(tcnt, tgen, k, epoch);
-      level++;
-    }
-    if (!near_far) break;
-    const int fn = ld_relaxed(&nf.cnt[fi * kLine]);
-    if (fn == 0) break;
-    int mn = INT32_MAX;
-    for (int i = rank * bd + (int)threadIdx.x; i < fn; i += k * bd) {
-      const int v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      mn = min(mn, (int)__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    mn = wave_min_int(mn);
-    if ((threadIdx.x & 63) == 0 && mn != INT32_MAX)
-      __hip_atomic_fetch_min(&nf.cnt[2 * kLine], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    team_barrier(tcnt, tgen, k, epoch);
-    thr = (uint32_t)ld_relaxed(&nf.cnt[2 * kLine]) + nf.delta;
-    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
-    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
-    for (int base = rank * bd; base < fn; base += k * bd) {
-      const int i = base + (int)threadIdx.x;
-      bool to_near = false, to_far = false;
-      int v = 0;
-      uint32_t dv = 0;
-      if (i < fn) {
-        v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        dv = __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (dv < thr) {
-          __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          to_near = true;
-        } else {
-          to_far = true;
-        }
-      }
-      push_item<true>(to_near, v, lq, &s.counts[(level % 3) * kLine], __HIP_MEMORY_SCOPE_AGENT, lqa, dv);
-      push_item<true>(to_far, v, nf.far(1 - fi), &nf.cnt[(1 - fi) * kLine], __HIP_MEMORY_SCOPE_AGENT);
-    }
-    team_barrier(tcnt, tgen, k, epoch);
-    // every member has read q[fi] and the minimum; the next split follows at
-    // least one more team barrier (the minimum moved to the near queue)
-    if (rank == 0 && threadIdx.x == 0) {
-      st_relaxed(&nf.cnt[fi * kLine], 0);
-      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
-    }
-    fi = 1 - fi;
-  }
-  if (rank == 0 && threadIdx.x == 0) *iters_out = level;
-}
 
 // PER_GRAPH: the same body instantiated for the one-launch-per-graph
 // baseline (a distinct symbol, so profiles separate the two modes).
@@ -664,8 +523,11 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     } else {
       int* tcnt = b.tbar + (size_t)j * 2 * kLine;
       int* tgen = tcnt + kLine;
-      if (sssp) run_team<true>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
-      else run_team<false>(g, s, nf, jb.src, r, jb.team, tcnt, tgen, lds, &b.iters[j]);
+      int epoch = 0;
+      const int team = jb.team;
+      auto bar = [&]() { team_barrier(tcnt, tgen, team, epoch); };
+      if (sssp) run_team<true>(g, s, nf, jb.src, r, jb.team, lds, &b.iters[j], bar);
+      else run_team<false>(g, s, nf, jb.src, r, jb.team, lds, &b.iters[j], bar);
     }
     // Thread 0 stamps the end before the loop's closing barrier: a thread-0
     // section after it would sit next to the ticket claim across the back

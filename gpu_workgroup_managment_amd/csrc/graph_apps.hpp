@@ -54,10 +54,101 @@ struct FrontierDev {
   // when set: edges relaxed (out-degrees of the live entries processed) are
   // added here, one atomic per frontier batch -- the traversal's work
   unsigned long long* relaxed;
+  // teams (COH) with heavy > 0 (the standalone apps, one team of every
+  // participant; the batched launch's teams of <= 16 gain nothing from it):
+  // an entry of more than `heavy` out-edges is
+  // deferred to the heavy list (vertex, distance, claim counter; fewer than
+  // m / heavy + 1 entries per level) and its edges are then claimed piece by
+  // piece by the whole team (frontier_heavy). hcnt: 3 lines, by level % 3.
+  int32_t* hv;
+  uint32_t* hdu;
+  int* hclaim;
+  int* hcnt;
+  int heavy;
 };
+
+// Heavy-entry threshold of a team with bd-thread workgroups: a workgroup's
+// frontier batch spreads up to about bd entries' edges evenly, so an entry
+// with several batches' worth of edges would hold back the workgroup that
+// claimed it. Never below kHeavyMin, which sizes the heavy lists.
+constexpr int kHeavyMin = 1024;
+__host__ __device__ constexpr int heavy_threshold(int bd) { return 4 * bd > kHeavyMin ? 4 * bd : kHeavyMin; }
 
 // Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17], claim[1]
 __host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 18) * 4; }
+
+// An empty register barrier: the compiler may not fold a later use of v back
+// into the conditional block that produced it, where it would wait for the
+// load or atomic before the next slot's is issued.
+__device__ __forceinline__ void opaque(uint32_t& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
+
+// Relax edge idx of a source at distance du (SSSP; BFS claims the target
+// for level `next`), then the wave pushes what improved: one counter atomic
+// per wave for the next level's queue (with distance tags under PAIRS) and
+// one for the far queue. Called by every lane of the wave; `active` marks
+// lanes with an edge.
+template <bool SSSP, bool LOCAL, bool COH, bool PAIRS>
+__device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev& s, bool active, int idx, uint32_t du,
+                                           int32_t* out_q, uint32_t* out_qa, int* out_cnt, uint32_t next,
+                                           uint32_t thr, int32_t* far_q, int* far_cnt, uint32_t* in_far) {
+  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
+  bool want = false, far_want = false;
+  int u = 0;
+  uint32_t alt = 0;
+  if (active) {
+    u = g.col[idx];
+    if (!SSSP) {
+      if (s.dist[u] == kInf) {
+        uint32_t expect = kInf;
+        want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                    scope);
+      }
+    } else {
+      alt = du + (uint32_t)g.wt[idx];
+      if (alt < s.dist[u]) {
+        const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
+        if (alt < old) {
+          if (alt < thr) want = PAIRS || __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+          else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
+        }
+      }
+    }
+  }
+  const unsigned long long mask = __ballot(want);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
+    base = __shfl(base, leader, 64);
+    if (want) {
+      const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+      if (COH) {
+        __hip_atomic_store(&out_q[pos], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (PAIRS) __hip_atomic_store(&out_qa[pos], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        out_q[pos] = u;
+        if (PAIRS) out_qa[pos] = alt;
+      }
+    }
+  }
+  if (SSSP && far_q) {
+    const unsigned long long fmask = __ballot(far_want);
+    if (fmask) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)fmask) - 1;
+      int base = 0;
+      if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
+      base = __shfl(base, leader, 64);
+      if (far_want) {
+        int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
+        if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *slot = u;
+      }
+    }
+  }
+}
 
 // Relax the edges of frontier entries in_q[a, b) (b - a <= blockDim.x).
 // The workgroup scans the entries' degrees and spreads their edges evenly
@@ -85,16 +176,17 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
                                                int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds,
                                                uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
                                                int* far_cnt = nullptr, uint32_t* in_far = nullptr,
-                                               const uint32_t* in_qa = nullptr, uint32_t* out_qa = nullptr) {
-  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
+                                               const uint32_t* in_qa = nullptr, uint32_t* out_qa = nullptr,
+                                               int* heavy_cnt = nullptr) {
   const int bd = blockDim.x, tid = threadIdx.x;
   int* s_end = lds;
   int* s_beg = lds + bd;
   uint32_t* s_du = (uint32_t*)(lds + 2 * bd);
   int* s_wave = lds + 3 * bd;
   const int i = a + tid;
-  int beg = 0, deg = 0;
+  int beg = 0, deg = 0, hv_v = 0;
   uint32_t du = 0;
+  bool hvy = false;
   if (i < b) {
     const int v = COH ? __hip_atomic_load(&in_q[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_q[i];
     beg = g.row_ptr[v];
@@ -103,6 +195,28 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
     if (PAIRS) {
       const uint32_t tag = COH ? __hip_atomic_load(&in_qa[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : in_qa[i];
       if (tag != du) deg = 0;  // stale: v was queued again with a lower distance
+    }
+    if (COH && heavy_cnt && deg > s.heavy) {
+      hvy = true;
+      hv_v = v;
+    }
+  }
+  if (COH && heavy_cnt) {  // defer heavy entries to the team-wide pass
+    const unsigned long long hm = __ballot(hvy);
+    if (hm) {
+      const int lane = tid & 63;
+      const int leader = __ffsll((long long)hm) - 1;
+      int base = 0;
+      if (lane == leader) base = __hip_atomic_fetch_add(heavy_cnt, __popcll(hm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      base = __shfl(base, leader, 64);
+      if (hvy) {
+        const int pos = base + __popcll(hm & ((1ull << lane) - 1ull));
+        __hip_atomic_store(&s.hv[pos], hv_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&s.hdu[pos], du, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&s.hclaim[pos], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s.relaxed) atomicAdd(s.relaxed, (unsigned long long)deg);
+        deg = 0;
+      }
     }
   }
   int total;
@@ -114,67 +228,19 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   __syncthreads();
   for (int eb = 0; eb < total; eb += bd) {
     const int e = eb + tid;
-    bool want = false, far_want = false;
-    int u = 0;
-    uint32_t alt = 0;
+    int idx = 0;
+    uint32_t du_e = 0;
     if (e < total) {
       int lo = 0, hi = bd - 1;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (s_end[mid] > e) hi = mid; else lo = mid + 1;
       }
-      const int idx = s_beg[lo] + e;
-      u = g.col[idx];
-      if (!SSSP) {
-        if (s.dist[u] == kInf) {
-          uint32_t expect = kInf;
-          want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                      scope);
-        }
-      } else {
-        alt = s_du[lo] + (uint32_t)g.wt[idx];
-        if (alt < s.dist[u]) {
-          const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
-          if (alt < old) {
-            if (alt < thr) want = PAIRS || __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
-            else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
-          }
-        }
-      }
+      idx = s_beg[lo] + e;
+      du_e = SSSP ? s_du[lo] : 0u;
     }
-    const unsigned long long mask = __ballot(want);
-    if (mask) {
-      const int lane = threadIdx.x & 63;
-      const int leader = __ffsll((long long)mask) - 1;
-      int base = 0;
-      if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
-      base = __shfl(base, leader, 64);
-      if (want) {
-        const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-        if (COH) {
-          __hip_atomic_store(&out_q[pos], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (PAIRS) __hip_atomic_store(&out_qa[pos], alt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-          out_q[pos] = u;
-          if (PAIRS) out_qa[pos] = alt;
-        }
-      }
-    }
-    if (SSSP && far_q) {
-      const unsigned long long fmask = __ballot(far_want);
-      if (fmask) {
-        const int lane = threadIdx.x & 63;
-        const int leader = __ffsll((long long)fmask) - 1;
-        int base = 0;
-        if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
-        base = __shfl(base, leader, 64);
-        if (far_want) {
-          int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
-          if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          else *slot = u;
-        }
-      }
-    }
+    relax_push<SSSP, LOCAL, COH, PAIRS>(g, s, e < total, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q, far_cnt,
+                                        in_far);
   }
   __syncthreads();
 }
@@ -207,6 +273,7 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
   uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
   int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
   const uint32_t next = (uint32_t)level + 1u;
+  int* heavy_cnt = COH && s.heavy > 0 ? &s.hcnt[(level % 3) * kLine] : nullptr;
   if (s.chunk > 0 && s.claims) {
     int* s_claim = lds + 3 * bd + 17;
     int* claim = &s.claims[(level % 3) * kLine];
@@ -219,15 +286,56 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
       const int c1 = min(c0 + s.chunk, in_n);
       for (int a = c0; a < c1; a += bd)
         frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr,
-                                                far_q, far_cnt, in_far, in_qa, out_qa);
+                                                far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
     }
     if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
   } else {
     for (int a = rank * bd; a < in_n; a += nteam * bd)
       frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr,
-                                              far_q, far_cnt, in_far, in_qa, out_qa);
+                                              far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
   }
-  if (rank == 0 && tid == 0) st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
+  if (rank == 0 && tid == 0) {
+    st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
+    // the heavy count of level - 2: every member read it (right after the
+    // barrier that ended level - 2's relaxation) before the barrier that
+    // ended level - 1's; level - 1's own count may still be being read
+    if (heavy_cnt) st_relaxed(&s.hcnt[((level + 1) % 3) * kLine], 0);
+  }
+}
+
+// Second pass of a team's level over the heavy entries frontier_batch
+// deferred: for each in turn, every workgroup claims bd edges at a time until
+// its out-edges are done, so a hub's edges spread over the whole team. Runs
+// between two team barriers; the pushes go to the same next-level queue.
+template <bool SSSP, bool PAIRS>
+__device__ __forceinline__ void frontier_heavy(const GraphDev& g, const FrontierDev& s, int level, int* lds,
+                                               uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
+                                               int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
+  const int bd = blockDim.x, tid = threadIdx.x;
+  int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+  uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
+  int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
+  const uint32_t next = (uint32_t)level + 1u;
+  int* s_claim = lds + 3 * bd + 17;
+  const int nh = __builtin_amdgcn_readfirstlane(ld_relaxed(&s.hcnt[(level % 3) * kLine]));
+  for (int h = 0; h < nh; h++) {
+    const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&s.hv[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const uint32_t du = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)__hip_atomic_load(&s.hdu[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int beg = __builtin_amdgcn_readfirstlane(g.row_ptr[v]);
+    const int deg = __builtin_amdgcn_readfirstlane(g.row_ptr[v + 1]) - beg;
+    while (true) {
+      if (tid == 0) *s_claim = __hip_atomic_fetch_add(&s.hclaim[h], bd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      const int c0 = __builtin_amdgcn_readfirstlane(*s_claim);
+      __syncthreads();
+      if (c0 >= deg) break;
+      const int e = c0 + tid;
+      relax_push<SSSP, false, true, PAIRS>(g, s, e < deg, beg + e, du, out_q, out_qa, out_cnt, next, thr, far_q,
+                                           far_cnt, in_far);
+      __syncthreads();  // the body ends in a barrier: the next claim by thread 0 stays one loop
+    }
+  }
 }
 
 // ----------------------------------------------------------- colouring

@@ -1,0 +1,164 @@
+// team_frontier.hpp -- BFS / SSSP over one graph by a team of workgroups
+// that share state through agent-scope atomics (the batched launch's teams
+// and the standalone frontier apps).
+#pragma once
+
+#include "graph_apps.hpp"
+
+namespace wgm {
+
+// Wave-aggregated push of `item` (and its distance tag into qa, when given)
+// where `want`, one counter atomic per wave.
+template <bool COH>
+__device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* cnt, int scope,
+                                          uint32_t* qa = nullptr, uint32_t tag = 0) {
+  const unsigned long long mask = __ballot(want);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) {
+    base = scope == __HIP_MEMORY_SCOPE_WORKGROUP
+               ? __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+               : __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  base = __shfl(base, leader, 64);
+  if (want) {
+    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+    if (COH) {
+      __hip_atomic_store(&q[pos], item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (qa) __hip_atomic_store(&qa[pos], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      q[pos] = item;
+      if (qa) qa[pos] = tag;
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_min_int(int x) {
+  for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
+// Counter barrier of a team of k workgroups (k small). Arrival is a fetch_add
+// on a monotonic counter; the arrival that completes epoch e publishes
+// generation e. Everything the team shares is accessed with agent-scope
+// atomics (frontier_batch<..., COH>), so the barrier only has to drain this
+// workgroup's outstanding accesses before arriving: no L2 write-back or
+// invalidation, which would also evict every other job's cached graph on the
+// XCD.
+__device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epoch) {
+  drain_vmem();
+  __syncthreads();
+  epoch++;
+  if (threadIdx.x == 0) {
+    const int old = add_relaxed(cnt, 1);
+    if (old == epoch * k - 1) {
+      st_relaxed(gen, epoch);
+    } else {
+      while (ld_relaxed(gen) < epoch) {
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// A graph served by a team of k workgroups (rank 0..k-1). Everything the
+// team shares -- queue entries and their distance tags, a frontier vertex's
+// distance, counters -- goes through agent-scope atomics, so `bar` (the
+// team's barrier) only has to drain, never to write back or invalidate an
+// L2. SSSP runs near-far bands of width nf.delta (0: frontier
+// Bellman-Ford), with the far counters and band minimum in device lines.
+// Reference: drelax2 (src/lonestar/graph_apps/bfs/device/bfs.cl:214-302,
+// src/lonestar/graph_apps/sssp/device/sssp.cl:262-318).
+template <bool SSSP, class Barrier>
+__device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int rank, int k,
+                         int* lds, int* iters_out, Barrier&& bar) {
+  const bool near_far = SSSP && nf.delta > 0;
+  int lo, hi;
+  block_range(g.n, rank, k, &lo, &hi);
+  // initial state through the coherent path (other members' atomics see it)
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    __hip_atomic_store(&s.dist[v], (v == src) ? 0u : kInf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (near_far) __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (rank == 0 && threadIdx.x == 0) {
+    __hip_atomic_store(&s.q0[0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (SSSP) __hip_atomic_store(&s.qa0[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st_relaxed(&s.counts[0], 1);
+    st_relaxed(&s.counts[kLine], 0);
+    st_relaxed(&s.counts[2 * kLine], 0);
+    for (int l = 0; l < 3; l++) {
+      st_relaxed(&s.claims[l * kLine], 0);
+      if (s.heavy > 0) st_relaxed(&s.hcnt[l * kLine], 0);
+    }
+    if (near_far) {
+      st_relaxed(&nf.cnt[0], 0);
+      st_relaxed(&nf.cnt[kLine], 0);
+      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
+    }
+  }
+  bar();
+  const int bd = blockDim.x;
+  int level = 0, fi = 0;
+  uint32_t thr = near_far ? nf.delta : 0xffffffffu;
+  while (true) {
+    while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
+      frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
+                                 nf.flag);
+      bar();
+      // heavy entries deferred by the pass (read after the barrier: every
+      // member sees the same count and takes the same number of barriers)
+      if (s.heavy > 0 && ld_relaxed(&s.hcnt[(level % 3) * kLine]) != 0) {
+        frontier_heavy<SSSP, SSSP>(g, s, level, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
+                                   nf.flag);
+        bar();
+      }
+      level++;
+    }
+    if (!near_far) break;
+    const int fn = ld_relaxed(&nf.cnt[fi * kLine]);
+    if (fn == 0) break;
+    int mn = INT32_MAX;
+    for (int i = rank * bd + (int)threadIdx.x; i < fn; i += k * bd) {
+      const int v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mn = min(mn, (int)__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    mn = wave_min_int(mn);
+    if ((threadIdx.x & 63) == 0 && mn != INT32_MAX)
+      __hip_atomic_fetch_min(&nf.cnt[2 * kLine], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bar();
+    thr = (uint32_t)ld_relaxed(&nf.cnt[2 * kLine]) + nf.delta;
+    int32_t* lq = (level & 1) ? s.q1 : s.q0;  // this level's (empty) queue
+    uint32_t* lqa = (level & 1) ? s.qa1 : s.qa0;
+    for (int base = rank * bd; base < fn; base += k * bd) {
+      const int i = base + (int)threadIdx.x;
+      bool to_near = false, to_far = false;
+      int v = 0;
+      uint32_t dv = 0;
+      if (i < fn) {
+        v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dv = __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (dv < thr) {
+          __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          to_near = true;
+        } else {
+          to_far = true;
+        }
+      }
+      push_item<true>(to_near, v, lq, &s.counts[(level % 3) * kLine], __HIP_MEMORY_SCOPE_AGENT, lqa, dv);
+      push_item<true>(to_far, v, nf.far(1 - fi), &nf.cnt[(1 - fi) * kLine], __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bar();
+    // every member has read q[fi] and the minimum; the next split follows at
+    // least one more team barrier (the minimum moved to the near queue)
+    if (rank == 0 && threadIdx.x == 0) {
+      st_relaxed(&nf.cnt[fi * kLine], 0);
+      st_relaxed(&nf.cnt[2 * kLine], INT32_MAX);
+    }
+    fi = 1 - fi;
+  }
+  if (rank == 0 && threadIdx.x == 0) *iters_out = level;
+}
+
+}  // namespace wgm

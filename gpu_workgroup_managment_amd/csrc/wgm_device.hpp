@@ -301,6 +301,10 @@ __device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n)
 // the leaders' writeback, readers on other XCDs see stale data) and an
 // L1-only invalidate (buffer_inv sc0) does not act outside threadgroup-split
 // mode, so members keep the full agent-scope acquire.
+// MAINT = false drops the write-back and the acquires: for callers whose
+// shared state goes only through agent-scope atomics (run_team), where the
+// barrier need only order arrivals after this workgroup's drained accesses.
+template <bool MAINT = true>
 __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
   drain_vmem();
   if (t.n <= 1) {
@@ -317,7 +321,7 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
       st_relaxed(f, arrive);
       while (ld_relaxed(f) != release) {
       }
-      fence_acquire();
+      if (MAINT) fence_acquire();
     }
   } else {
     for (int r = 1 + (int)threadIdx.x; r < t.xm; r += blockDim.x) {
@@ -327,7 +331,7 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      fence_release();
+      if (MAINT) fence_release();
       st_relaxed(&c.xflags[t.xcc * kLine], t.epoch);
     }
     if (threadIdx.x < kMaxXcd && ld_relaxed(&c.xcd_count[threadIdx.x * kLine]) > 0) {
@@ -335,7 +339,7 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) fence_acquire();
+    if (MAINT && threadIdx.x == 0) fence_acquire();
     __syncthreads();
     for (int r = 1 + (int)threadIdx.x; r < t.xm; r += blockDim.x) st_relaxed(&c.flags[members[r] * kLine], release);
   }
