@@ -113,3 +113,52 @@ def test_pagerank_grid(rt, side, wgs):
     l1 = float(np.abs(r.astype(np.float64) - ref64).sum())
     print(f"PageRank {side}^2: L1 vs FP64 = {l1:.3e}")
     assert l1 < 1e-4
+
+
+def _hub_chain(levels=6, leaves=(3000, 6000), seed=5, wmax=256):
+    """Two hubs per level, each pointing at its own block of leaves; every leaf
+    points at both hubs of the next level, plus random edges. Each frontier
+    then holds several entries far above the heavy threshold (4 * bd, at
+    least 1024 edges) next to many small ones."""
+    rng = np.random.default_rng(seed)
+    hubs = np.arange(2 * levels).reshape(levels, 2)
+    nxt = 2 * levels
+    src, dst = [], []
+    for lv in range(levels):
+        for k in range(2):
+            cnt = leaves[k]
+            blk = np.arange(nxt, nxt + cnt)
+            nxt += cnt
+            src.append(np.full(cnt, hubs[lv, k]))
+            dst.append(blk)
+            if lv + 1 < levels:
+                for h in hubs[lv + 1]:
+                    src.append(blk)
+                    dst.append(np.full(cnt, h))
+    n = nxt
+    extra = 4 * n
+    src.append(rng.integers(2 * levels, n, extra))
+    dst.append(rng.integers(0, n, extra))
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    wt = rng.integers(1, wmax, src.size).astype(np.int32)
+    return G.from_edges(n, src, dst, wt)
+
+
+@pytest.mark.parametrize("wg", [64, 256, 1024])
+def test_frontier_heavy_entries(rt, wg):
+    """Entries of thousands of edges are deferred and their edges claimed in
+    pieces by every workgroup; BFS levels and SSSP distances stay exact, the
+    latter with weights large enough for many near-far bands."""
+    g = _hub_chain()
+    assert g.degrees().max() > 4 * 1024
+    dg = rt.graph(g)
+    dist, st = dg.bfs(0, wg_size=wg, num_wgs=16)
+    ref, levels = oracle.bfs(g.row_ptr, g.col, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["iterations"] == levels
+    g = _hub_chain(wmax=20000)
+    dg = rt.graph(g)
+    dist, st = dg.sssp(0, wg_size=wg, num_wgs=16)
+    ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())
