@@ -136,8 +136,12 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
     if (!near_far) break;
     const int fn = s_far[fi];
     if (fn == 0) break;
+    const uint32_t t_old = thr;  // far entries below it were relaxed in the band (dropped)
     int mn = INT32_MAX;
-    for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.far(fi)[i]]);
+    for (int i = threadIdx.x; i < fn; i += bd) {
+      const uint32_t d = s.dist[nf.far(fi)[i]];
+      if (d >= t_old) mn = min(mn, (int)d);
+    }
     mn = wave_min_int(mn);
     if ((threadIdx.x & 63) == 0) atomicMin(&s_far[2], mn);
     __syncthreads();
@@ -154,7 +158,7 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
         dv = s.dist[v];
         if (dv < thr) {
           nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
-          to_near = true;
+          to_near = dv >= t_old;
         } else {
           to_far = true;  // stays flagged in the other far queue
         }
@@ -373,12 +377,13 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
     if (fn == 0) break;
     const int p = level & 1;  // this level's (empty) queue
     // move a far entry below the new threshold to the near queue, else to the other far queue
+    const uint32_t t_old = thr;  // far entries below it were relaxed in the band (dropped)
     auto split = [&](bool in, int v, uint32_t dv, uint32_t rv) {
       bool to_near = false, to_far = false;
       if (in) {
         if (dv < thr) {
           nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
-          to_near = true;
+          to_near = dv >= t_old;
         } else {
           to_far = true;  // stays flagged in the other far queue
         }
@@ -404,14 +409,17 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         dv = s.dist[v];
         rv = row_info(g, v);
       }
-      const int mn = wave_min_int(in ? (int)dv : INT32_MAX);
+      const int mn = wave_min_int(in && dv >= t_old ? (int)dv : INT32_MAX);
       if (lane == 0) atomicMin(&s_far[2], mn);
       __syncthreads();
       thr = (uint32_t)__builtin_amdgcn_readfirstlane(s_far[2]) + nf.delta;
       split(in, v, dv, rv);
     } else {
       int mn = INT32_MAX;
-      for (int i = threadIdx.x; i < fn; i += bd) mn = min(mn, (int)s.dist[nf.far(fi)[i]]);
+      for (int i = threadIdx.x; i < fn; i += bd) {
+        const uint32_t d = s.dist[nf.far(fi)[i]];
+        if (d >= t_old) mn = min(mn, (int)d);
+      }
       mn = wave_min_int(mn);
       if (lane == 0) atomicMin(&s_far[2], mn);
       __syncthreads();

@@ -247,6 +247,11 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
 
 // Near-far state of one traversal: two far queues (n + 1 entries each) and
 // their counters, the in_far flags, the band minimum and width.
+// A band split drops the far entries whose distance is already below the
+// finished band's threshold: any improvement below the threshold queues the
+// vertex near, so such a vertex was relaxed at that distance in the band
+// (moving it back to near would relax its edges twice), and the band minimum
+// is taken over the rest, which all lie at or above that threshold.
 struct NearFar {
   int32_t* q[2];
   // select, not index: a runtime index into q[] puts the pair in scratch

@@ -120,9 +120,11 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
     const int fn = ld_relaxed(&nf.cnt[fi * kLine]);
     if (fn == 0) break;
     int mn = INT32_MAX;
+    const uint32_t t_old = thr;  // far entries below it were relaxed in the band (dropped)
     for (int i = rank * bd + (int)threadIdx.x; i < fn; i += k * bd) {
       const int v = __hip_atomic_load(&nf.far(fi)[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      mn = min(mn, (int)__hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      const uint32_t d = __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (d >= t_old) mn = min(mn, (int)d);
     }
     mn = wave_min_int(mn);
     if ((threadIdx.x & 63) == 0 && mn != INT32_MAX)
@@ -141,7 +143,7 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
         dv = __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (dv < thr) {
           __hip_atomic_store(&nf.flag[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          to_near = true;
+          to_near = dv >= t_old;
         } else {
           to_far = true;
         }
