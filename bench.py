@@ -24,8 +24,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Tuned on MI355X over this workload (wg 64..1024, 1..8 workgroups per CU,
 # team granularity 2^14..2^22 edges, chunk 32..4096, SSSP near-far band
-# 0..16384): 1024 x 2 per CU, teams of ceil(m / 2^20), band 1024.
-TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=1024)
+# 0..16384 on the grids and 4..1024 on the RMAT graphs): 1024 x 2 per CU
+# (one fits), teams of ceil(m / 2^20), band 512 on graphs of out-degree <= 4
+# and 8 on the others.
+TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
+             delta_high_degree=8)
 
 
 def parse():
@@ -353,14 +356,15 @@ def main():
     gteps = edges / (ms_per_step * 1e6)
 
     # Baseline: one persistent launch per graph, each graph's launch as wide
-    # as its team. The best of three team granularities is reported (the
+    # as its team. The best of three team granularities, each with the
+    # batched bands and with one 1024 band for every graph, is reported (the
     # batched shape's coarse teams would understate it).
     relaunch, relaunch_tried = None, []
-    for shape in (dict(cfg_d), dict(cfg_d, team_edges=1 << 16, max_team=16), dict(cfg_d, team_edges=1 << 14,
-                                                                                   max_team=64)):
+    shapes = (dict(cfg_d), dict(cfg_d, team_edges=1 << 16, max_team=16), dict(cfg_d, team_edges=1 << 14, max_team=64))
+    for shape in shapes + tuple(dict(sh, delta=1024, delta_high_degree=0) for sh in shapes):
         for _ in range(args.relaunch_steps):
             r = batch.run_relaunch(B.config(**shape))
-            r["shape"] = {"team_edges": shape["team_edges"], "max_team": shape["max_team"]}
+            r["shape"] = {k: shape.get(k, 0) for k in ("team_edges", "max_team", "delta", "delta_high_degree")}
             relaunch_tried.append({"gteps": round(r["gteps"], 4), **r["shape"]})
             relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
     # the relaunch overwrote results: run the batched launch once more for the checks below

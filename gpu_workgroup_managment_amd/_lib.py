@@ -129,6 +129,7 @@ class BatchConfig(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("delta", ctypes.c_int32),
         ("sssp_team_edges", ctypes.c_int32),
+        ("delta_high_degree", ctypes.c_int32),
     ]
 
     def as_dict(self):

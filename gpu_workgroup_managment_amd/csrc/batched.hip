@@ -58,7 +58,7 @@ struct BatchJob {
   int index;          // caller's graph index
   int maxdeg;         // largest out-degree (picks the degree-bounded frontier step)
   int qcap;           // entries per queue: n + 1 (BFS), max(n, m) + 1 (SSSP, one entry per relaxation)
-  int pad;
+  uint32_t delta;     // SSSP near-far band width of this query (0: off)
 };
 
 struct BatchDev {
@@ -82,7 +82,6 @@ struct BatchDev {
   int32_t* far;    // near-far: far queues
   uint32_t* in_far;
   int* fcnt;       // 3 lines per job: far counts and band minimum
-  uint32_t delta;  // near-far band width (0: off)
   unsigned long long* jt;  // per job: first member's start, last member's end (wall clock)
   unsigned long long* reached;  // per job: out-degrees of the reached vertices, summed after the traversal
   unsigned long long* relaxed;  // per job: edges relaxed from live frontier entries (work)
@@ -511,7 +510,7 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     s.relaxed = b.relaxed + j;
     const bool sssp = jb.query == WGM_QUERY_SSSP;
     NearFar nf{};
-    nf.delta = sssp && b.far ? b.delta : 0u;
+    nf.delta = sssp && b.far ? jb.delta : 0u;
     if (nf.delta) {
       nf.q[0] = b.far + jb.f_off;
       nf.q[1] = b.far + jb.f_off + jb.n + 1;
@@ -647,7 +646,6 @@ struct wgm_batch {
   int32_t* fcnt = nullptr;        // 3 lines per job
   int64_t total_f = 0;
   std::vector<int64_t> f_off;
-  uint32_t delta = 0;             // near-far band of the current job table
   BatchJob* d_jobs = nullptr;
   // per-job results and control snapshot in one block (one copy per run):
   // reached[count], jt[2 count], relaxed[count], snap[4], then iters (int32)[count]
@@ -708,7 +706,6 @@ BatchDev dev_view(wgm_batch* b) {
   d.far = b->far;
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
-  d.delta = b->delta;
   return d;
 }
 
@@ -722,6 +719,7 @@ int normalize(wgm_runtime* rt, const wgm_batch_config* in, wgm_batch_config* out
   if (c.chunk <= 0) c.chunk = c.wg_size;
   if (c.delta < 0) c.delta = 0;
   if (c.sssp_team_edges < 0) c.sssp_team_edges = 0;
+  if (c.delta_high_degree < 0) c.delta_high_degree = 0;
   int rc = wgm_check_wg(rt, c.wg_size);
   if (rc) return rc;
   if (c.wgs_per_cu > 32 || c.max_team > 1024) return fail(rt, WGM_ERR_INVALID, "bad batch config");
@@ -761,10 +759,10 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.index = i;
     j.maxdeg = b->maxdeg[i];
     j.qcap = b->qcap[i];
+    j.delta = (uint32_t)(b->maxdeg[i] > kLowDegree && cfg.delta_high_degree > 0 ? cfg.delta_high_degree : cfg.delta);
     ticket += j.team;
   }
   b->chunk = cfg.chunk;
-  b->delta = (uint32_t)cfg.delta;
   b->jobs_cfg = cfg;
   b->jobs_grid = grid;
   WGM_HIPCHK(b->rt, hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * b->count, hipMemcpyHostToDevice));
@@ -1167,11 +1165,27 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
           }
         }
     }
-  // then the SSSP near-far band at the best launch shape
+  // then the SSSP near-far band at the best launch shape: the band of
+  // low-degree graphs, then that of the others
   const int deltas[] = {0, 512, 2048, 4096};
   for (int dl : deltas) {
     wgm_batch_config c = best;
     c.delta = dl;
+    for (int r = 0; r < reps; r++) {
+      wgm_batch_stats s{};
+      int rc = wgm_batch_run(rt, b, &c, &s);
+      if (rc) return rc;
+      if (s.wall_ms < best_ms) {
+        best_ms = s.wall_ms;
+        best = c;
+        bst = s;
+      }
+    }
+  }
+  const int deltas_hi[] = {4, 8, 32, 128};
+  for (int dl : deltas_hi) {
+    wgm_batch_config c = best;
+    c.delta_high_degree = dl;
     for (int r = 0; r < reps; r++) {
       wgm_batch_stats s{};
       int rc = wgm_batch_run(rt, b, &c, &s);

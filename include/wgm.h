@@ -507,6 +507,9 @@ typedef struct wgm_batch_config {
   int32_t delta;       /* SSSP near-far band width (0: plain frontier Bellman-Ford) */
   int32_t sssp_team_edges; /* team granularity of SSSP queries (0: team_edges); an SSSP edge
                               costs two to three BFS edges, so its teams can be larger */
+  int32_t delta_high_degree; /* SSSP band width on graphs with an out-degree above 4 (0: delta):
+                                their few wide levels favour narrow bands (less rework), the
+                                long narrow traversals of low-degree graphs wide ones (fewer levels) */
 } wgm_batch_config;
 
 typedef struct wgm_batch_stats {

@@ -38,6 +38,31 @@ def test_batched_persistent(rt, small_batch, team_edges, chunk, delta):
     b.close()
 
 
+@pytest.mark.parametrize("delta,delta_high_degree", [(0, 1), (1024, 1), (512, 8)])
+def test_batched_band_per_degree_class(rt, small_batch, delta, delta_high_degree):
+    """Graphs with an out-degree above 4 take their own band width (with
+    delta 0 only they run near-far): exact either way. At width 1 every band
+    is one distance and weights are >= 1, so each reached vertex's edges are
+    relaxed exactly once, at its final distance."""
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    b.run(wg_size=256, wgs_per_cu=2, team_edges=512, max_team=8, delta=delta, delta_high_degree=delta_high_degree)
+    _check_all(b, graphs, refs)
+    b.close()
+    hi = [i for i, g in enumerate(graphs) if queries[i] == L.QUERY_SSSP and g.degrees().max() > 4]
+    assert hi
+    sub = B.Batch(rt, [graphs[i] for i in hi], queries[hi])
+    st = sub.run(wg_size=256, wgs_per_cu=2, team_edges=512, max_team=8, delta=delta,
+                 delta_high_degree=delta_high_degree)
+    for k, i in enumerate(hi):
+        assert np.array_equal(sub.result(k)[0], refs[i])
+    if delta_high_degree == 1:
+        assert st["edges_relaxed"] == st["edges_scanned"]
+    else:
+        assert st["edges_relaxed"] >= st["edges_scanned"]
+    sub.close()
+
+
 def test_batched_relaunch(rt, small_batch):
     graphs, queries, refs = small_batch
     b = B.Batch(rt, graphs, queries)
