@@ -7,6 +7,7 @@ struct wgm_graph {
   wgm_runtime* rt = nullptr;
   int32_t n = 0;
   int64_t m = 0;
+  int32_t maxdeg = 0;  // largest out-degree
   int32_t* row_ptr = nullptr;
   int32_t* col = nullptr;
   int32_t* wt = nullptr;
