@@ -25,6 +25,7 @@
 // polling the slots' records.
 
 #include "sched_internal.hpp"
+#include "team_frontier.hpp"
 
 #include <atomic>
 #include <cmath>
@@ -43,7 +44,15 @@ struct ServeGraphDev {
   const int32_t* wt;
   int n;
   int m;
+  uint32_t delta;  // SSSP near-far band width
+  int pad;
 };
+
+// SSSP band widths of the server's queries, as tuned for the batched launch
+// (weights in [1, 255]): wide on graphs of out-degree <= 4 (long, narrow
+// traversals), narrow on the others (short, wide ones).
+constexpr uint32_t kServeDelta = 512;
+constexpr uint32_t kServeDeltaHighDegree = 8;
 
 // Written by the host into its slot before the slot id is published.
 struct ServeReq {
@@ -64,6 +73,8 @@ struct ServeRec {
 };
 static_assert(sizeof(ServeRec) == 64, "completion record: one 64-byte line per slot");
 
+constexpr int kSaved = 5;  // ints saved per slot at a yield
+
 struct ServeDev {
   const ServeGraphDev* graphs;
   const ServeReq* reqs;        // host memory, device view
@@ -77,7 +88,7 @@ struct ServeDev {
   int* resume;                 // [slots]: slot + 1, 0 when empty
   int* resume_head;            // device line
   int* resume_tail;            // device line
-  int* saved;                  // [slots * 2]: level, frontier size
+  int* saved;                  // [slots * kSaved]: level, frontier size, band threshold, far queue, its count
   long long* saved_t0;         // [slots]: device clock at the first level
   int* saved_pre;              // [slots]: times yielded
   uint32_t* arena;             // request state
@@ -157,10 +168,20 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 
 // One query on this workgroup, from the top or from a saved level. Returns
 // true if the workgroup yielded (the query is in the resume queue).
+// SSSP runs near-far bands as the batched single-workgroup step does: a
+// relaxation at or past the band's threshold defers the target to the far
+// queue (once, under its flag); when the near frontier empties, the band
+// moves to the smallest deferred distance still at or above the old
+// threshold + delta, and the far queue is split. The request state is dist,
+// stamps, two frontier queues and, for SSSP, two far queues and the flags.
+// A yield happens only at a level's end, with the band's threshold, far queue
+// and its count saved beside the level.
 template <bool SSSP>
 __device__ bool serve_query(const SchedDev& s, const ServeDev& v, int slot, bool resume, const ServeReq& rq,
                             int* lds, int* s_cnt, int* s_misc) {
   const ServeGraphDev gd = v.graphs[rq.graph];
+  __shared__ int s_far[3];        // far counts of far(0), far(1); band minimum
+  __shared__ uint32_t s_band[2];  // threshold, current far queue
   GraphDev g{};
   g.row_ptr = gd.row;
   g.col = gd.col;
@@ -172,10 +193,19 @@ __device__ bool serve_query(const SchedDev& s, const ServeDev& v, int slot, bool
   f.stamp = f.dist + gd.n;
   f.q0 = (int32_t*)(f.stamp + gd.n);
   f.q1 = f.q0 + gd.n + 1;
+  const bool near_far = SSSP && gd.delta > 0;
+  NearFar nf{};
+  if (near_far) {
+    nf.q[0] = f.q1 + gd.n + 1;
+    nf.q[1] = nf.q[0] + gd.n + 1;
+    nf.flag = (uint32_t*)(nf.q[1] + gd.n + 1);
+    nf.delta = gd.delta;
+  }
   if (!resume) {
     for (int x = threadIdx.x; x < gd.n; x += blockDim.x) {
       f.dist[x] = (x == rq.src) ? 0u : kInf;
       if (SSSP) f.stamp[x] = 0u;
+      if (near_far) nf.flag[x] = 0u;
     }
     if (threadIdx.x == 0) {
       f.q0[0] = rq.src;
@@ -183,47 +213,109 @@ __device__ bool serve_query(const SchedDev& s, const ServeDev& v, int slot, bool
       s_cnt[1] = 0;
       s_cnt[2] = 0;
       s_misc[0] = 0;
+      s_far[0] = 0;
+      s_far[1] = 0;
+      s_far[2] = INT32_MAX;
+      s_band[0] = near_far ? nf.delta : 0xffffffffu;
+      s_band[1] = 0u;
       v.saved_t0[slot] = (long long)wall_clock64();
       v.saved_pre[slot] = 0;
     }
   } else if (threadIdx.x == 0) {
-    const int level = v.saved[2 * slot];
-    s_cnt[level % 3] = v.saved[2 * slot + 1];
+    const int* sv = v.saved + (size_t)slot * kSaved;
+    const int level = sv[0];
+    s_cnt[level % 3] = sv[1];
     s_cnt[(level + 1) % 3] = 0;
     s_cnt[(level + 2) % 3] = 0;
     s_misc[0] = level;
+    s_band[0] = (uint32_t)sv[2];
+    s_band[1] = (uint32_t)sv[3];
+    s_far[sv[3]] = sv[4];
+    s_far[1 - sv[3]] = 0;
+    s_far[2] = INT32_MAX;
   }
   drain_vmem();
   __syncthreads();
   int level = s_misc[0];
+  uint32_t thr = s_band[0];
+  int fi = (int)s_band[1];
   const int bd = blockDim.x;
   while (true) {
-    const int in_n = s_cnt[level % 3];
-    if (in_n == 0) break;
-    const int32_t* in_q = (level & 1) ? f.q1 : f.q0;
-    int32_t* out_q = (level & 1) ? f.q0 : f.q1;
-    for (int a = 0; a < in_n; a += bd)
-      frontier_batch<SSSP, true>(g, f, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
-                                 (uint32_t)level + 1u, lds);
-    if (threadIdx.x == 0) {
-      s_cnt[(level + 2) % 3] = 0;
-      s_misc[1] = take_kill(s) ? 1 : 0;
+    while (true) {
+      const int in_n = s_cnt[level % 3];
+      if (in_n == 0) break;
+      const int32_t* in_q = (level & 1) ? f.q1 : f.q0;
+      int32_t* out_q = (level & 1) ? f.q0 : f.q1;
+      for (int a = 0; a < in_n; a += bd)
+        frontier_batch<SSSP, true>(g, f, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
+                                   (uint32_t)level + 1u, lds, thr, near_far ? nf.far(fi) : nullptr, &s_far[fi],
+                                   nf.flag);
+      if (threadIdx.x == 0) {
+        s_cnt[(level + 2) % 3] = 0;
+        s_misc[1] = take_kill(s) ? 1 : 0;
+      }
+      drain_vmem();
+      __syncthreads();
+      level++;
+      if (s_misc[1]) {
+        // yield: the level is complete, its successor's frontier is in out_q
+        if (threadIdx.x == 0) {
+          int* sv = v.saved + (size_t)slot * kSaved;
+          sv[0] = level;
+          sv[1] = s_cnt[level % 3];
+          sv[2] = (int)thr;
+          sv[3] = fi;
+          sv[4] = s_far[fi];
+          v.saved_pre[slot] += 1;
+          fence_release();
+          push_resume(v, slot);
+        }
+        __syncthreads();
+        return true;
+      }
     }
+    if (!near_far) break;
+    const int fn = s_far[fi];
+    if (fn == 0) break;
+    // band split: far entries already below the finished band's threshold
+    // were queued near when they fell below it (relaxed in the band) and are
+    // dropped; the new minimum is over the rest
+    const uint32_t t_old = thr;
+    int mn = INT32_MAX;
+    for (int i = threadIdx.x; i < fn; i += bd) {
+      const uint32_t d = f.dist[nf.far(fi)[i]];
+      if (d >= t_old) mn = min(mn, (int)d);
+    }
+    mn = wave_min_int(mn);
+    if ((threadIdx.x & 63) == 0) atomicMin(&s_far[2], mn);
+    __syncthreads();
+    thr = (uint32_t)s_far[2] + nf.delta;
+    int32_t* lq = (level & 1) ? f.q1 : f.q0;  // this level's (empty) queue
+    for (int base = 0; base < fn; base += bd) {
+      const int i = base + threadIdx.x;
+      bool to_near = false, to_far = false;
+      int x = 0;
+      if (i < fn) {
+        x = nf.far(fi)[i];
+        const uint32_t dx = f.dist[x];
+        if (dx < thr) {
+          nf.flag[x] = 0u;  // each deferred vertex is in one far queue once: it moves once
+          to_near = dx >= t_old;
+        } else {
+          to_far = true;  // stays flagged in the other far queue
+        }
+      }
+      push_item<false>(to_near, x, lq, &s_cnt[level % 3], __HIP_MEMORY_SCOPE_WORKGROUP);
+      push_item<false>(to_far, x, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_far[fi] = 0;
+      s_far[2] = INT32_MAX;
+    }
+    fi = 1 - fi;
     drain_vmem();
     __syncthreads();
-    level++;
-    if (s_misc[1]) {
-      // yield: the level is complete, its successor's frontier is in out_q
-      if (threadIdx.x == 0) {
-        v.saved[2 * slot] = level;
-        v.saved[2 * slot + 1] = s_cnt[level % 3];
-        v.saved_pre[slot] += 1;
-        fence_release();
-        push_resume(v, slot);
-      }
-      __syncthreads();
-      return true;
-    }
   }
   // completion: digest, reached vertices and scanned edges in one pass
   fence_acquire();
@@ -491,7 +583,12 @@ struct wgm_serve {
 
 namespace {
 
-int64_t state_words(int n) { return (((int64_t)4 * n + 2 + 63) / 64) * 64; }
+// Request state: dist, stamps and two frontier queues (4 n + 2 words); SSSP
+// adds two far queues and the far flags (3 n + 2).
+int64_t state_words(int n, bool sssp) {
+  const int64_t w = (int64_t)4 * n + 2 + (sssp ? (int64_t)3 * n + 2 : 0);
+  return (w + 63) / 64 * 64;
+}
 
 // Harvest finished slots, then admit queued requests.
 void serve_progress(wgm_serve* s) {
@@ -524,7 +621,7 @@ void serve_progress(wgm_serve* s) {
   const int64_t room = s->cfg.budget_bytes - s->graph_bytes;
   while (!s->pending.empty() && !s->free_slots.empty()) {
     const Pending& p = s->pending.front();
-    const int64_t bytes = state_words(s->gn[p.graph]) * 4;
+    const int64_t bytes = state_words(s->gn[p.graph], p.query == WGM_QUERY_SSSP) * 4;
     if (s->inflight_bytes + bytes > room) break;  // admission: wait for memory to free up
     const int slot = s->free_slots.back();
     s->free_slots.pop_back();
@@ -630,7 +727,10 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     HIPCHK(rt, hipMemcpy(base + off_row[i], g.row_ptr, sizeof(int32_t) * (g.n + 1), hipMemcpyHostToDevice));
     if (g.m > 0) HIPCHK(rt, hipMemcpy(base + off_col[i], g.col, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
     if (g.wt && g.m > 0) HIPCHK(rt, hipMemcpy(base + off_wt[i], g.wt, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
-    tab[i] = ServeGraphDev{base + off_row[i], base + off_col[i], g.wt ? base + off_wt[i] : nullptr, g.n, (int)g.m};
+    int32_t maxdeg = 0;
+    for (int32_t x = 0; x < g.n; x++) maxdeg = std::max(maxdeg, g.row_ptr[x + 1] - g.row_ptr[x]);
+    tab[i] = ServeGraphDev{base + off_row[i], base + off_col[i], g.wt ? base + off_wt[i] : nullptr, g.n, (int)g.m,
+                           maxdeg > 4 ? kServeDeltaHighDegree : kServeDelta, 0};
   }
   e = hipMalloc(&s->d_gtab, sizeof(ServeGraphDev) * count);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(graph table)"));
@@ -641,10 +741,10 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   // device lines + per-slot arrays + state arena
   const int slots = cfg.max_inflight;
   const size_t line_b = kLine * sizeof(int);
-  const size_t lines_bytes = 8 * line_b + (size_t)slots * (sizeof(int) * 4 + sizeof(long long));
+  const size_t lines_bytes = 8 * line_b + (size_t)slots * (sizeof(int) * (2 + kSaved) + sizeof(long long));
   e = hipMalloc(&s->d_lines, lines_bytes);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(server lines)"));
-  s->slot_words = state_words(s->nmax);
+  s->slot_words = state_words(s->nmax, true);
   e = hipMalloc(&s->d_arena, (size_t)slots * s->slot_words * 4);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(request state)"));
   // host memory: reqs, recs (64-byte stride), ring (2 * slots), tail
@@ -683,7 +783,7 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   v.resume = (int*)p;
   p += (size_t)slots * sizeof(int);
   v.saved = (int*)p;
-  p += (size_t)slots * 2 * sizeof(int);
+  p += (size_t)slots * kSaved * sizeof(int);
   v.saved_pre = (int*)p;
   v.arena = (uint32_t*)s->d_arena;
   s->inflight.assign(slots, Inflight{});
@@ -770,7 +870,7 @@ static int serve_submit_at(wgm_serve* s, int32_t graph, int32_t query, int32_t s
   if (query != WGM_QUERY_BFS && query != WGM_QUERY_SSSP) return fail(rt, WGM_ERR_INVALID, "unknown query %d", query);
   if (query == WGM_QUERY_SSSP && !s->has_wt[graph]) return fail(rt, WGM_ERR_INVALID, "graph %d has no weights", graph);
   if (src < 0 || src >= s->gn[graph]) return fail(rt, WGM_ERR_INVALID, "bad source %d", src);
-  const int64_t bytes = state_words(s->gn[graph]) * 4;
+  const int64_t bytes = state_words(s->gn[graph], query == WGM_QUERY_SSSP) * 4;
   if (s->graph_bytes + bytes > s->cfg.budget_bytes)
     return fail(rt, WGM_ERR_BUDGET, "request state (%lld bytes) can never fit the budget", (long long)bytes);
   Pending p{s->next_id++, graph, query, src, t_arrival};

@@ -408,7 +408,9 @@ int wgm_mega_send_task_custom(wgm_mega* m, int32_t groups, wgm_task_times* out);
  * published through a ring in coherent host memory and run concurrently,
  * one workgroup per query. A query yields its workgroup at any level
  * boundary when the scheduler needs workgroups for a non-persistent task and
- * resumes on another; freed workgroups rejoin the server. */
+ * resumes on another; freed workgroups rejoin the server. A request's state,
+ * charged to the budget, is 4n + 2 words for BFS and 7n + 4 for SSSP (its
+ * near-far queues and flags), each rounded up to 64 words. */
 typedef struct wgm_serve wgm_serve;
 
 typedef struct wgm_serve_config {
