@@ -246,7 +246,9 @@ typedef struct wgm_run_stats {
  * wgm_bfs: Lonestar BFS (src/lonestar/graph_apps/bfs/bfs.h:16-370,
  *   src/lonestar/graph_apps/bfs/device/standalone.cl:218-320);
  * wgm_sssp: Lonestar SSSP (src/lonestar/graph_apps/sssp/sssp.h:16-382,
- *   src/lonestar/graph_apps/sssp/device/standalone.cl);
+ *   src/lonestar/graph_apps/sssp/device/standalone.cl), here in near-far
+ *   bands of width 16 on graphs with an out-degree above 4 and 1024 on the
+ *   others (same distances; the reference relaxes in plain frontier order);
  * wgm_color: Pannotia color (src/pannotia/graph_apps/color/color.h:41-195,
  *   src/pannotia/graph_apps/color/device/standalone.cl:10-126);
  * wgm_mis: Pannotia MIS (src/pannotia/graph_apps/mis/mis.h:40-202,
@@ -545,8 +547,9 @@ int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* i
 int wgm_batch_job_times(const wgm_batch* b, double* start_us, double* end_us);
 /* Grid search over wg_size {64..1024}, wgs_per_cu {1,2,4,8}, team_edges
  * {2^14..2^20} and chunk {32..4096} (band 1024), then the SSSP near-far band
- * {0, 512..4096} and the SSSP team granularity {2^16..2^19} at the best
- * shape; returns the fastest configuration. */
+ * {0, 512..4096}, the band of graphs with an out-degree above 4 {4..128} and
+ * the SSSP team granularity {2^16..2^19} at the best shape; returns the
+ * fastest configuration. */
 int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best,
                        wgm_batch_stats* best_stats);
 
