@@ -116,6 +116,30 @@ def test_serve_admission_budget(rt, pool):
     assert len(recs) == len(gs)
 
 
+def test_serve_admission_budget_sssp(rt, pool):
+    """An SSSP request also holds its near-far queues and flags: 7n + 4 words
+    (rounded up to 64), so a budget sized for two of the largest SSSP
+    requests admits at most two at a time and every result is still exact."""
+    gs, want = pool
+    probe = S.Server(rt, gs, S.config(max_inflight=8))
+    gbytes = probe.graph_bytes
+    probe.close()
+    nmax = max(g.n for g in gs)
+    state = ((7 * nmax + 4 + 63) // 64) * 64 * 4
+    budget = gbytes + 2 * state
+    with S.Server(rt, gs, S.config(max_inflight=32, budget_bytes=budget)) as srv:
+        srv.start()
+        peak = 0
+        for i in range(len(gs)):
+            srv.submit(i, S.QUERY_SSSP)
+            peak = max(peak, srv.inflight_bytes())
+        recs = srv.drain()
+        srv.stop()
+    assert peak <= budget - gbytes
+    _check(recs, want)
+    assert len(recs) == len(gs) and all(r["query"] == S.QUERY_SSSP for r in recs)
+
+
 def test_serve_rejects_what_never_fits(rt, pool):
     gs, _ = pool
     probe = S.Server(rt, gs, S.config(max_inflight=8))
