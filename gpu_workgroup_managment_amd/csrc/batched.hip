@@ -35,7 +35,11 @@ namespace wgm {
 // teams for the 1M-edge grids the batch took 17.4 ms instead of 10.2 ms.
 constexpr int kLowDegree = 4;
 constexpr long long kLowDegreeSingle = 1ll << 22;
-constexpr int kLowSlots = 4;  // frontier entries per thread group in flight together
+// Frontier entries per thread group in flight together. A narrow level's
+// cost is instruction issue, and each unrolled slot adds its scalar checks
+// to every level: on the 512-query batch, 4 slots took 8.99 ms per step,
+// 2 took 8.75 and 1 took 8.64 (levels past bd / K entries take more passes).
+constexpr int kLowSlots = 1;
 
 // The low-degree step keeps the first low_lds_cap(bd) entries of each of its
 // two frontier queues (vertex, row info, distance tag) in LDS behind the
