@@ -361,7 +361,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             int base = 0;
             if (lane == leader)
               base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            base = __shfl(base, leader, 64);
+            base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
             if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP);
           }
           if (SSSP) {
@@ -398,7 +398,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         if (lane == leader)
           nb = __hip_atomic_fetch_add(&s_cnt[level % 3], __popcll(mask), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-        nb = __shfl(nb, leader, 64);
+        nb = __builtin_amdgcn_readlane(nb, leader);
         if (to_near) lq.store(p, nb + __popcll(mask & below), v, rv, dv, true);
       }
       push_item<false>(to_far, v, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -121,7 +121,7 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
     const int leader = __ffsll((long long)mask) - 1;
     int base = 0;
     if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, scope);
-    base = __shfl(base, leader, 64);
+    base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
     if (want) {
       const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
       if (COH) {
@@ -140,7 +140,7 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
       const int leader = __ffsll((long long)fmask) - 1;
       int base = 0;
       if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, __popcll(fmask), __ATOMIC_RELAXED, scope);
-      base = __shfl(base, leader, 64);
+      base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
       if (far_want) {
         int32_t* slot = &far_q[base + __popcll(fmask & ((1ull << lane) - 1ull))];
         if (COH) __hip_atomic_store(slot, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -208,7 +208,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
       const int leader = __ffsll((long long)hm) - 1;
       int base = 0;
       if (lane == leader) base = __hip_atomic_fetch_add(heavy_cnt, __popcll(hm), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      base = __shfl(base, leader, 64);
+      base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
       if (hvy) {
         const int pos = base + __popcll(hm & ((1ull << lane) - 1ull));
         __hip_atomic_store(&s.hv[pos], hv_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

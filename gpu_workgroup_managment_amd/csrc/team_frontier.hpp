@@ -22,7 +22,7 @@ __device__ __forceinline__ void push_item(bool want, int item, int32_t* q, int* 
                ? __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                : __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  base = __shfl(base, leader, 64);
+  base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
   if (want) {
     const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
     if (COH) {

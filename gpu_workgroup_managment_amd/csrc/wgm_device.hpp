@@ -450,7 +450,7 @@ __device__ __forceinline__ void wave_push(bool want, int item, int* queue, int* 
   const int leader = __ffsll((long long)mask) - 1;
   int base = 0;
   if (lane == leader) base = add_relaxed(count, __popcll(mask));
-  base = __shfl(base, leader, 64);
+  base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
   if (want) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = item;
 }
 
