@@ -63,6 +63,24 @@ def test_batched_band_per_degree_class(rt, small_batch, delta, delta_high_degree
     sub.close()
 
 
+def test_batched_autotune_returns_an_exact_config(rt, small_batch):
+    """The autotuner's choice is one of the shapes and bands it sweeps
+    (workgroup size, band of the low-degree graphs, band of the others) and
+    serves every query exactly."""
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    best, st = b.autotune(reps=1)
+    cfg = best.as_dict()
+    print(cfg, st["wall_ms"])
+    assert cfg["wg_size"] in (64, 128, 256, 512, 1024)
+    assert cfg["delta"] in (0, 512, 1024, 2048, 4096)
+    assert cfg["delta_high_degree"] in (0, 4, 8, 32, 128)
+    assert st["wall_ms"] > 0
+    b.run(best)
+    _check_all(b, graphs, refs)
+    b.close()
+
+
 def test_batched_relaunch(rt, small_batch):
     graphs, queries, refs = small_batch
     b = B.Batch(rt, graphs, queries)
