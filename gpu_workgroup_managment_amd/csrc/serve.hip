@@ -744,7 +744,9 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   const size_t lines_bytes = 8 * line_b + (size_t)slots * (sizeof(int) * (2 + kSaved) + sizeof(long long));
   e = hipMalloc(&s->d_lines, lines_bytes);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(server lines)"));
-  s->slot_words = state_words(s->nmax, true);
+  // a slot holds any request; SSSP (the larger state) needs a weighted graph
+  const bool any_wt = std::find(s->has_wt.begin(), s->has_wt.end(), (char)1) != s->has_wt.end();
+  s->slot_words = state_words(s->nmax, any_wt);
   e = hipMalloc(&s->d_arena, (size_t)slots * s->slot_words * 4);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(request state)"));
   // host memory: reqs, recs (64-byte stride), ring (2 * slots), tail

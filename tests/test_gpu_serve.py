@@ -140,6 +140,28 @@ def test_serve_admission_budget_sssp(rt, pool):
     assert len(recs) == len(gs) and all(r["query"] == S.QUERY_SSSP for r in recs)
 
 
+def test_serve_unweighted_graphs_bfs_only(rt):
+    """A server whose graphs carry no weights sizes its request slots for
+    BFS, serves BFS exactly and refuses SSSP."""
+    gs = [G.grid(48, 40), G.grid(64, 64)]
+    with S.Server(rt, gs, S.config(wg_size=256, max_inflight=8)) as srv:
+        srv.start()
+        for i in range(len(gs)):
+            srv.submit(i, S.QUERY_BFS)
+        with pytest.raises(Exception) as e:
+            srv.submit(0, S.QUERY_SSSP)
+        assert "weights" in str(e.value)
+        recs = srv.drain()
+        srv.stop()
+    assert len(recs) == len(gs)
+    for r in recs:
+        g = gs[r["graph"]]
+        b = oracle.bfs(g.row_ptr, g.col, 0)[0]
+        reached = b != oracle.INF
+        assert (r["digest"], r["reached"], r["edges_scanned"]) == (
+            S.dist_digest(b), int(reached.sum()), int(g.degrees()[reached].sum()))
+
+
 def test_serve_rejects_what_never_fits(rt, pool):
     gs, _ = pool
     probe = S.Server(rt, gs, S.config(max_inflight=8))
