@@ -272,6 +272,9 @@ SIGNATURES = {
     "wgm_mega_get_stats": (_i32, [_vp, _P(MegaStats)]),
     "wgm_mega_result": (_i32, [_vp, _vp, _P(_i32)]),
     "wgm_np_run_solo": (_i32, [_vp, _i32, _i32, _i32, _P(ctypes.c_double), _P(_i32)]),
+    "wgm_np_copy": (_i32, [_vp, _i32, _i32, _vp, _i64]),
+    "wgm_np_result": (_i32, [_vp, _P(_i32), _P(ctypes.c_double)]),
+    "wgm_np_expected": (_i32, [_vp, _i32, _P(_i32), _P(ctypes.c_double), _P(ctypes.c_double)]),
     "wgm_args_create": (_i32, [_P(_vp)]),
     "wgm_args_push": (_i32, [_vp, _vp, _i32]),
     "wgm_args_destroy": (_i32, [_vp]),
@@ -282,6 +285,8 @@ SIGNATURES = {
     "wgm_serve_create": (_i32, [_vp, _i32, _P(Csr), _P(ServeConfig), _P(_vp)]),
     "wgm_serve_destroy": (_i32, [_vp]),
     "wgm_serve_graph_bytes": (_i64, [_vp]),
+    "wgm_serve_plan": (_i32, [_i32, _P(Csr), _P(ServeConfig), _P(_i64), _P(_i64)]),
+    "wgm_serve_arena_bytes": (_i64, [_vp]),
     "wgm_serve_start": (_i32, [_vp, _P(_i32)]),
     "wgm_serve_submit": (_i32, [_vp, _i32, _i32, _i32, _P(_i64)]),
     "wgm_serve_poll": (_i32, [_vp, _P(ServeDone), _i32, _P(_i32)]),

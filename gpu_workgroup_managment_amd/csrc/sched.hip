@@ -623,6 +623,52 @@ int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels) {
   return WGM_OK;
 }
 
+// The tasks' resident buffers, copied to the host so that a caller can check
+// them against its own restatement (the reference's host keeps its copies of
+// A, B and C for check_non_persistent_task, matmult.h:120-149,
+// reduce.h:18-35). The resident kernel must have quit: a copy may need the GPU.
+int wgm_np_copy(wgm_mega* m, int32_t kind, int32_t which, void* host, int64_t bytes) {
+  if (!m || !host) return fail(m ? m->rt : nullptr, WGM_ERR_INVALID, "NULL argument");
+  wgm_runtime* rt = m->rt;
+  if (m->running) return fail(rt, WGM_ERR_STATE, "copy task buffers after wgm_mega_quit");
+  const NPDev& t = m->np;
+  const void* src = nullptr;
+  int64_t want = (int64_t)t.dim * t.dim * 4;
+  if (kind == WGM_NP_MATMULT_I32) src = which == WGM_NP_IN_A ? (const void*)t.A : which == WGM_NP_IN_B ? (const void*)t.B
+                                        : which == WGM_NP_OUT_C ? (const void*)t.C : nullptr;
+  else if (kind == WGM_NP_MATMUL_F32) src = which == WGM_NP_IN_A ? (const void*)t.Af : which == WGM_NP_IN_B ? (const void*)t.Bf
+                                        : which == WGM_NP_OUT_C ? (const void*)t.Cf : nullptr;
+  else if (kind == WGM_NP_REDUCE_MIN && which == WGM_NP_IN_A) {
+    src = t.buf;
+    want = (int64_t)t.len * 4;
+  }
+  if (!src) return fail(rt, WGM_ERR_INVALID, "task kind %d has no buffer %d", kind, which);
+  if (bytes != want) return fail(rt, WGM_ERR_INVALID, "buffer is %lld bytes, %lld asked", (long long)want, (long long)bytes);
+  HIPCHK(rt, hipMemcpy(host, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return WGM_OK;
+}
+
+// The summary the last task published (djb2 hash, minimum, or the FP32
+// checksum with value = 1), and the values the library checks it against.
+int wgm_np_result(wgm_mega* m, int32_t* value, double* checksum) {
+  if (!m) return fail(nullptr, WGM_ERR_INVALID, "NULL mega");
+  if (value) *value = hload(m->h_np_result);
+  if (checksum) {
+    unsigned long long bits = __hip_atomic_load(m->h_np_checksum, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    std::memcpy(checksum, &bits, 8);
+  }
+  return WGM_OK;
+}
+
+int wgm_np_expected(wgm_mega* m, int32_t kind, int32_t* value, double* checksum, double* tol) {
+  if (!m) return fail(nullptr, WGM_ERR_INVALID, "NULL mega");
+  if (kind < WGM_NP_MATMULT_I32 || kind > WGM_NP_REDUCE_MIN) return fail(m->rt, WGM_ERR_INVALID, "unknown task kind %d", kind);
+  if (value) *value = kind == WGM_NP_MATMULT_I32 ? m->np_expect_hash : kind == WGM_NP_REDUCE_MIN ? m->np_expect_min : 1;
+  if (checksum) *checksum = kind == WGM_NP_MATMUL_F32 ? m->np_expect_sum : 0.0;
+  if (tol) *tol = kind == WGM_NP_MATMUL_F32 ? m->np_tol : 0.0;
+  return WGM_OK;
+}
+
 // run_non_persistent (src/host_skeleton/skel.cpp:168-221): the task alone,
 // `iterations` launches of wg_size x num_wgs, each checked.
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us, int32_t* errors) {

@@ -384,8 +384,12 @@ __device__ __forceinline__ void scheduler_loop(const SchedDev& s, int lpg) {
         __hip_atomic_store(&s.np_clock[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if (s.serve) {
-        // publish: k tickets, then the epoch (one release for the whole task)
+        // publish: k tickets, then the epoch. A free workgroup whose epoch is
+        // stale may take a ticket as soon as the ticket counter is reset, before
+        // the epoch moves, so the task's kind, counters and k are released
+        // before that reset (and the taker acquires after its ticket RMW).
         st_relaxed(s.np_k, k);
+        fence_release();
         st_relaxed(s.np_ticket, 0);
         fence_release();
         add_relaxed(s.np_epoch, 1);

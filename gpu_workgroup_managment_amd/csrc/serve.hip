@@ -31,6 +31,8 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -463,6 +465,7 @@ __global__ void __launch_bounds__(1024) k_serve(Ctl c, SchedDev s, ServeDev v, N
         if (e != seen_epoch) {
           seen_epoch = e;
           const int tk = add_relaxed(s.np_ticket, 1);
+          fence_acquire();  // pairs with the release before the ticket reset: np_k / np_kind are the new task's
           if (tk < ld_relaxed(s.np_k)) {
             add_relaxed(s.available, -1);
             t = TASK_MULT;
@@ -498,8 +501,7 @@ __global__ void k_serve_reset(ServeDev v) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int i = t; i < v.slots; i += gridDim.x * blockDim.x) {
     v.resume[i] = 0;
-    v.saved[2 * i] = 0;
-    v.saved[2 * i + 1] = 0;
+    for (int k = 0; k < kSaved; k++) v.saved[i * kSaved + k] = 0;
     v.saved_t0[i] = 0;
     v.saved_pre[i] = 0;
   }
@@ -537,9 +539,20 @@ struct Inflight {
   int graph = 0, query = 0, src = 0, seq = 0;
   uint64_t t_submit = 0, t_publish = 0;
   int64_t bytes = 0;
+  int64_t off = 0;  // 4-byte words into the state arena
 };
 
 int serve_lds(int wg) { return serve_lds_bytes(wg); }
+
+wgm_serve_config serve_defaults(const wgm_serve_config* in) {
+  wgm_serve_config cfg = in ? *in : wgm_serve_config{};
+  if (cfg.wg_size <= 0) cfg.wg_size = 1024;
+  if (cfg.max_inflight <= 0) cfg.max_inflight = 256;
+  if (cfg.reserve == 0) cfg.reserve = 16;
+  if (cfg.reserve < 0) cfg.reserve = 0;
+  if (cfg.budget_bytes <= 0) cfg.budget_bytes = (int64_t)64 << 30;
+  return cfg;
+}
 
 }  // namespace
 
@@ -553,11 +566,12 @@ struct wgm_serve {
   std::vector<char> has_wt;
   int64_t graph_bytes = 0;
   int nmax = 0;
-  int64_t slot_words = 0;
+  int64_t arena_words = 0;   // request-state arena: graphs + arena <= budget
+  std::map<int64_t, int64_t> free_ranges;  // free arena words: offset -> length, coalesced
   void* d_graphs = nullptr;  // pooled CSR arrays
   void* d_gtab = nullptr;    // ServeGraphDev[count]
   void* d_lines = nullptr;   // lines + per-slot arrays
-  void* d_arena = nullptr;   // slots * slot_words
+  void* d_arena = nullptr;   // arena_words
   char* h_mem = nullptr;     // reqs, recs, ring, tail (coherent host memory)
   char* h_mem_dev = nullptr;
   ServeReq* h_reqs = nullptr;
@@ -590,6 +604,37 @@ int64_t state_words(int n, bool sssp) {
   return (w + 63) / 64 * 64;
 }
 
+// Request state is carved out of the arena first-fit; a freed range merges
+// with its free neighbours. With nothing in flight the arena is one free
+// range, so any request that fits the arena at all is admitted eventually.
+int64_t arena_take(wgm_serve* s, int64_t words) {
+  for (auto it = s->free_ranges.begin(); it != s->free_ranges.end(); ++it) {
+    if (it->second < words) continue;
+    const int64_t off = it->first, len = it->second;
+    s->free_ranges.erase(it);
+    if (len > words) s->free_ranges[off + words] = len - words;
+    return off;
+  }
+  return -1;
+}
+
+void arena_give(wgm_serve* s, int64_t off, int64_t words) {
+  auto next = s->free_ranges.lower_bound(off);
+  if (next != s->free_ranges.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      off = prev->first;
+      words += prev->second;
+      s->free_ranges.erase(prev);
+    }
+  }
+  if (next != s->free_ranges.end() && off + words == next->first) {
+    words += next->second;
+    s->free_ranges.erase(next);
+  }
+  s->free_ranges[off] = words;
+}
+
 // Harvest finished slots, then admit queued requests.
 void serve_progress(wgm_serve* s) {
   const uint64_t now = now_ns();
@@ -615,14 +660,16 @@ void serve_progress(wgm_serve* s) {
     d.preempted = r.preempted;
     s->done.push_back(d);
     s->inflight_bytes -= f.bytes;
+    arena_give(s, f.off, f.bytes / 4);
     f.id = -1;
     s->free_slots.push_back(slot);
   }
-  const int64_t room = s->cfg.budget_bytes - s->graph_bytes;
   while (!s->pending.empty() && !s->free_slots.empty()) {
     const Pending& p = s->pending.front();
-    const int64_t bytes = state_words(s->gn[p.graph], p.query == WGM_QUERY_SSSP) * 4;
-    if (s->inflight_bytes + bytes > room) break;  // admission: wait for memory to free up
+    const int64_t words = state_words(s->gn[p.graph], p.query == WGM_QUERY_SSSP);
+    const int64_t off = arena_take(s, words);
+    if (off < 0) break;  // admission: wait for memory to free up
+    const int64_t bytes = words * 4;
     const int slot = s->free_slots.back();
     s->free_slots.pop_back();
     Inflight& f = s->inflight[slot];
@@ -633,6 +680,7 @@ void serve_progress(wgm_serve* s) {
     f.seq = ++s->seq;
     f.t_submit = p.t_submit;
     f.bytes = bytes;
+    f.off = off;
     s->inflight_bytes += bytes;
     if (s->inflight_bytes > s->max_inflight_bytes) s->max_inflight_bytes = s->inflight_bytes;
     ServeReq rq{};
@@ -640,7 +688,7 @@ void serve_progress(wgm_serve* s) {
     rq.query = p.query;
     rq.src = p.src;
     rq.seq = f.seq;
-    rq.state_off = (long long)slot * s->slot_words;
+    rq.state_off = (long long)off;
     std::memcpy((void*)&s->h_reqs[slot], &rq, sizeof(rq));
     s->h_ring[s->tail % s->dev.ring_cap] = slot;
     s->tail++;
@@ -648,6 +696,42 @@ void serve_progress(wgm_serve* s) {
     hstore(s->h_ring_tail, s->tail);
     s->pending.pop_front();
   }
+}
+
+struct ServePlan {
+  int64_t graph_words = 0;  // pooled CSR arrays
+  int64_t arena_words = 0;  // request-state arena
+  int nmax = 0;
+};
+
+// Memory plan: the graph pool (per graph row_ptr, col and wt if present, each
+// 256-byte aligned) and the request-state arena, as large as max_inflight of
+// the largest request but never beyond the budget left by the graphs.
+// Returns an error message, or nullptr.
+const char* serve_plan(int count, const wgm_csr* graphs, const wgm_serve_config& cfg, ServePlan* out,
+                       int64_t* off_row, int64_t* off_col, int64_t* off_wt) {
+  auto align = [](int64_t w) { return (w + 63) / 64 * 64; };
+  ServePlan p;
+  bool any_wt = false;
+  for (int i = 0; i < count; i++) {
+    const wgm_csr& g = graphs[i];
+    if (g.n < 1 || g.m < 0 || g.m > INT32_MAX || !g.row_ptr || (g.m > 0 && !g.col)) return "a graph is malformed";
+    p.nmax = std::max(p.nmax, g.n);
+    any_wt = any_wt || g.wt != nullptr;
+    if (off_row) off_row[i] = p.graph_words;
+    p.graph_words += align((int64_t)g.n + 1);
+    if (off_col) off_col[i] = p.graph_words;
+    p.graph_words += align(std::max<int64_t>(g.m, 1));
+    if (off_wt) off_wt[i] = g.wt ? p.graph_words : -1;
+    if (g.wt) p.graph_words += align(std::max<int64_t>(g.m, 1));
+  }
+  *out = p;
+  if (p.graph_words * 4 > cfg.budget_bytes) return "the resident graphs exceed the budget";
+  // SSSP (the larger state) needs a weighted graph
+  const int64_t largest = state_words(p.nmax, any_wt);
+  const int64_t room = (cfg.budget_bytes / 4 - p.graph_words) / 64 * 64;
+  out->arena_words = std::min<int64_t>((int64_t)cfg.max_inflight * largest, room);
+  return nullptr;
 }
 
 }  // namespace
@@ -678,12 +762,7 @@ uint64_t wgm_dist_digest(const uint32_t* dist, int32_t n) {
 int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const wgm_serve_config* cfg_in,
                      wgm_serve** out) {
   if (!rt || !graphs || !out || count < 1) return fail(rt, WGM_ERR_INVALID, "bad arguments");
-  wgm_serve_config cfg = cfg_in ? *cfg_in : wgm_serve_config{};
-  if (cfg.wg_size <= 0) cfg.wg_size = 1024;
-  if (cfg.max_inflight <= 0) cfg.max_inflight = 256;
-  if (cfg.reserve == 0) cfg.reserve = 16;
-  if (cfg.reserve < 0) cfg.reserve = 0;
-  if (cfg.budget_bytes <= 0) cfg.budget_bytes = (int64_t)64 << 30;
+  const wgm_serve_config cfg = serve_defaults(cfg_in);
   int rc = wgm_check_wg(rt, cfg.wg_size);
   if (rc) return rc;
   if (cfg.wg_size < 128) return fail(rt, WGM_ERR_INVALID, "the server needs wg_size >= 128");
@@ -692,41 +771,29 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   s->cfg = cfg;
   s->count = count;
   auto bail = [&](int code) { wgm_serve_destroy(s); return code; };
-  // graph pool: per graph row_ptr (n+1), col (m), wt (m, if present), 256-byte aligned
   std::vector<int64_t> off_row(count), off_col(count), off_wt(count);
-  int64_t words = 0;
-  auto align = [](int64_t w) { return (w + 63) / 64 * 64; };
+  ServePlan plan;
+  const char* why = serve_plan(count, graphs, cfg, &plan, off_row.data(), off_col.data(), off_wt.data());
+  if (why) return bail(fail(rt, plan.graph_words * 4 > cfg.budget_bytes ? WGM_ERR_BUDGET : WGM_ERR_INVALID, "%s", why));
   for (int i = 0; i < count; i++) {
-    const wgm_csr& g = graphs[i];
-    if (g.n < 1 || g.m < 0 || g.m > INT32_MAX || !g.row_ptr || (g.m > 0 && !g.col))
-      return bail(fail(rt, WGM_ERR_INVALID, "graph %d is malformed", i));
-    s->gn.push_back(g.n);
-    s->gm.push_back(g.m);
-    s->has_wt.push_back(g.wt != nullptr);
-    s->nmax = std::max(s->nmax, g.n);
-    off_row[i] = words;
-    words += align(g.n + 1);
-    off_col[i] = words;
-    words += align(std::max<int64_t>(g.m, 1));
-    off_wt[i] = -1;
-    if (g.wt) {
-      off_wt[i] = words;
-      words += align(std::max<int64_t>(g.m, 1));
-    }
+    s->gn.push_back(graphs[i].n);
+    s->gm.push_back(graphs[i].m);
+    s->has_wt.push_back(graphs[i].wt != nullptr);
   }
+  s->nmax = plan.nmax;
+  const int64_t words = plan.graph_words;
   s->graph_bytes = words * 4;
-  if (s->graph_bytes > cfg.budget_bytes)
-    return bail(fail(rt, WGM_ERR_BUDGET, "resident graphs (%lld bytes) exceed the budget (%lld)",
-                     (long long)s->graph_bytes, (long long)cfg.budget_bytes));
   hipError_t e = hipMalloc(&s->d_graphs, (size_t)words * 4);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(graph pool)"));
   std::vector<ServeGraphDev> tab(count);
   for (int i = 0; i < count; i++) {
     const wgm_csr& g = graphs[i];
     int32_t* base = (int32_t*)s->d_graphs;
-    HIPCHK(rt, hipMemcpy(base + off_row[i], g.row_ptr, sizeof(int32_t) * (g.n + 1), hipMemcpyHostToDevice));
-    if (g.m > 0) HIPCHK(rt, hipMemcpy(base + off_col[i], g.col, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
-    if (g.wt && g.m > 0) HIPCHK(rt, hipMemcpy(base + off_wt[i], g.wt, sizeof(int32_t) * g.m, hipMemcpyHostToDevice));
+    e = hipMemcpy(base + off_row[i], g.row_ptr, sizeof(int32_t) * (g.n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g.m > 0) e = hipMemcpy(base + off_col[i], g.col, sizeof(int32_t) * g.m, hipMemcpyHostToDevice);
+    if (e == hipSuccess && g.wt && g.m > 0)
+      e = hipMemcpy(base + off_wt[i], g.wt, sizeof(int32_t) * g.m, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMemcpy(graph pool)"));
     int32_t maxdeg = 0;
     for (int32_t x = 0; x < g.n; x++) maxdeg = std::max(maxdeg, g.row_ptr[x + 1] - g.row_ptr[x]);
     tab[i] = ServeGraphDev{base + off_row[i], base + off_col[i], g.wt ? base + off_wt[i] : nullptr, g.n, (int)g.m,
@@ -734,7 +801,8 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   }
   e = hipMalloc(&s->d_gtab, sizeof(ServeGraphDev) * count);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(graph table)"));
-  HIPCHK(rt, hipMemcpy(s->d_gtab, tab.data(), sizeof(ServeGraphDev) * count, hipMemcpyHostToDevice));
+  e = hipMemcpy(s->d_gtab, tab.data(), sizeof(ServeGraphDev) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMemcpy(graph table)"));
   // the scheduler cells and the non-persistent task buffers
   rc = wgm_mega_create(rt, &cfg.np, &s->mega);
   if (rc) return bail(rc);
@@ -744,11 +812,13 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   const size_t lines_bytes = 8 * line_b + (size_t)slots * (sizeof(int) * (2 + kSaved) + sizeof(long long));
   e = hipMalloc(&s->d_lines, lines_bytes);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(server lines)"));
-  // a slot holds any request; SSSP (the larger state) needs a weighted graph
-  const bool any_wt = std::find(s->has_wt.begin(), s->has_wt.end(), (char)1) != s->has_wt.end();
-  s->slot_words = state_words(s->nmax, any_wt);
-  e = hipMalloc(&s->d_arena, (size_t)slots * s->slot_words * 4);
-  if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(request state)"));
+  // request state: the planned arena, carved per admitted request
+  s->arena_words = plan.arena_words;
+  if (s->arena_words > 0) {
+    e = hipMalloc(&s->d_arena, (size_t)s->arena_words * 4);
+    if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(request state)"));
+    s->free_ranges[0] = s->arena_words;
+  }
   // host memory: reqs, recs (64-byte stride), ring (2 * slots), tail
   const int ring_cap = 2 * slots;
   const size_t h_bytes = (size_t)slots * 32 + (size_t)slots * 64 + (size_t)ring_cap * 4 + line_b;
@@ -808,6 +878,19 @@ int wgm_serve_destroy(wgm_serve* s) {
 }
 
 int64_t wgm_serve_graph_bytes(const wgm_serve* s) { return s ? s->graph_bytes : 0; }
+int64_t wgm_serve_arena_bytes(const wgm_serve* s) { return s ? s->arena_words * 4 : 0; }
+
+int wgm_serve_plan(int32_t count, const wgm_csr* graphs, const wgm_serve_config* cfg_in, int64_t* graph_bytes,
+                   int64_t* arena_bytes) {
+  if (!graphs || count < 1) return fail(nullptr, WGM_ERR_INVALID, "bad arguments");
+  wgm_serve_config cfg = serve_defaults(cfg_in);
+  ServePlan plan;
+  const char* why = serve_plan(count, graphs, cfg, &plan, nullptr, nullptr, nullptr);
+  if (graph_bytes) *graph_bytes = plan.graph_words * 4;
+  if (arena_bytes) *arena_bytes = plan.arena_words * 4;
+  if (why) return fail(nullptr, plan.graph_words * 4 > cfg.budget_bytes ? WGM_ERR_BUDGET : WGM_ERR_INVALID, "%s", why);
+  return WGM_OK;
+}
 int32_t wgm_serve_outstanding(const wgm_serve* s) { return s ? (int32_t)s->outstanding : 0; }
 int64_t wgm_serve_inflight_bytes(const wgm_serve* s) { return s ? s->inflight_bytes : 0; }
 
@@ -873,8 +956,9 @@ static int serve_submit_at(wgm_serve* s, int32_t graph, int32_t query, int32_t s
   if (query == WGM_QUERY_SSSP && !s->has_wt[graph]) return fail(rt, WGM_ERR_INVALID, "graph %d has no weights", graph);
   if (src < 0 || src >= s->gn[graph]) return fail(rt, WGM_ERR_INVALID, "bad source %d", src);
   const int64_t bytes = state_words(s->gn[graph], query == WGM_QUERY_SSSP) * 4;
-  if (s->graph_bytes + bytes > s->cfg.budget_bytes)
-    return fail(rt, WGM_ERR_BUDGET, "request state (%lld bytes) can never fit the budget", (long long)bytes);
+  if (bytes > s->arena_words * 4)
+    return fail(rt, WGM_ERR_BUDGET, "request state (%lld bytes) can never fit the budget next to the graphs",
+                (long long)bytes);
   Pending p{s->next_id++, graph, query, src, t_arrival};
   s->pending.push_back(p);
   s->outstanding++;

@@ -110,6 +110,31 @@ class Mega:
         self.rt._check(L.lib().wgm_np_run_solo(self._h, wg_size, num_wgs, iterations, ctypes.byref(us), ctypes.byref(errs)))
         return us.value, errs.value
 
+    def np_copy(self, kind, which, dim=256, length=1 << 20):
+        """A task's resident buffer (NP_IN_A / NP_IN_B / NP_OUT_C) as a numpy
+        array: int32 or float32 dim x dim, or the reduction's int32 input."""
+        if kind == NP_REDUCE_MIN:
+            out = np.empty(length, np.int32)
+        else:
+            out = np.empty((dim, dim), np.float32 if kind == NP_MATMUL_F32 else np.int32)
+        self.rt._check(L.lib().wgm_np_copy(self._h, kind, which, L.ptr(out), out.nbytes))
+        return out
+
+    def np_result(self):
+        """(value, checksum) the last task published."""
+        v, c = ctypes.c_int32(), ctypes.c_double()
+        self.rt._check(L.lib().wgm_np_result(self._h, ctypes.byref(v), ctypes.byref(c)))
+        return v.value, c.value
+
+    def np_expected(self, kind):
+        """(value, checksum, tolerance) the library checks a task of `kind` against."""
+        v, c, t = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        self.rt._check(L.lib().wgm_np_expected(self._h, kind, ctypes.byref(v), ctypes.byref(c), ctypes.byref(t)))
+        return v.value, c.value, t.value
+
+
+NP_IN_A, NP_IN_B, NP_OUT_C = 0, 1, 2
+
 
 def np_groups_for(ratio, occupancy):
     """get_workgroups_for_non_persistent (src/host_skeleton/skel.cpp:387-401):

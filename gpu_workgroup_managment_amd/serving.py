@@ -26,6 +26,18 @@ def config(wg_size=1024, num_wgs=0, max_inflight=256, reserve=16, budget_bytes=6
     return c
 
 
+def plan(graphs, cfg=None):
+    """(graph_bytes, arena_bytes) of a server on `graphs` (host only): the
+    request-state arena never takes more than the budget the graphs leave."""
+    arr = (L.Csr * len(graphs))()
+    for i, g in enumerate(graphs):
+        arr[i] = g.as_c()
+    cfg = cfg if cfg is not None else config()
+    gb, ab = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib().wgm_serve_plan(len(graphs), arr, ctypes.byref(cfg), ctypes.byref(gb), ctypes.byref(ab)))
+    return gb.value, ab.value
+
+
 class Server:
     """wgm_serve: create uploads the graphs; start launches the resident kernel."""
 
@@ -62,6 +74,10 @@ class Server:
     @property
     def graph_bytes(self):
         return int(L.lib().wgm_serve_graph_bytes(self._h))
+
+    @property
+    def arena_bytes(self):
+        return int(L.lib().wgm_serve_arena_bytes(self._h))
 
     def start(self):
         p = ctypes.c_int32()

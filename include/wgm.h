@@ -370,6 +370,25 @@ int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
                     int32_t* errors);
 
+/* The non-persistent tasks' resident buffers, copied to the host (after
+ * wgm_mega_quit; `bytes` must be the buffer's size): WGM_NP_IN_A is A (int32
+ * or FP32, dim x dim) or the reduction's `length` int32 values, WGM_NP_IN_B is
+ * B, WGM_NP_OUT_C the last product C. The reference's host keeps the same
+ * buffers to check a task (matmult.h:120-149, reduce.h:18-35). */
+#define WGM_NP_IN_A 0
+#define WGM_NP_IN_B 1
+#define WGM_NP_OUT_C 2
+int wgm_np_copy(wgm_mega* m, int32_t kind, int32_t which, void* host, int64_t bytes);
+/* The summary the last task published to the host (check_non_persistent_task,
+ * matmult.h:190-194): the djb2 hash of C's diagonal (WGM_NP_MATMULT_I32), the
+ * minimum (WGM_NP_REDUCE_MIN) or 1 with *checksum = sum_ij C[i][j] u(i) v(j),
+ * u(i) = i % 7 + 1, v(j) = j % 5 + 1, in FP64 (WGM_NP_MATMUL_F32). */
+int wgm_np_result(wgm_mega* m, int32_t* value, double* checksum);
+/* The values the library checks that summary against, computed on the host
+ * when the inputs were made (FP32: the checksum u^T A . B v in FP64 and its
+ * a-priori FP32 error bound). */
+int wgm_np_expected(wgm_mega* m, int32_t kind, int32_t* value, double* checksum, double* tol);
+
 /* --------------------------------------------- the caller's own kernels
  * Persistent kernels written against include/wgm_device_api.h (the
  * reference's scheduler API: kernel_merge/stubs/stubs.h:4-7,
@@ -446,6 +465,16 @@ int wgm_serve_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
 int wgm_serve_destroy(wgm_serve* s);
 /* Resident graph bytes (charged to the budget). */
 int64_t wgm_serve_graph_bytes(const wgm_serve* s);
+/* Memory plan of a server, computed on the host without a GPU: the graph pool
+ * and the request-state arena. The arena is max_inflight times the largest
+ * request's state, cut to the budget the graphs leave, so graph_bytes +
+ * arena_bytes <= budget_bytes; every admitted request's state is carved out of
+ * it (first fit) and returned when the request finishes. WGM_ERR_BUDGET if the
+ * graphs alone exceed the budget. */
+int wgm_serve_plan(int32_t count, const wgm_csr* graphs, const wgm_serve_config* cfg, int64_t* graph_bytes,
+                   int64_t* arena_bytes);
+/* Bytes of the request-state arena the server allocated (its plan). */
+int64_t wgm_serve_arena_bytes(const wgm_serve* s);
 /* Launch the resident kernel and start the server on all workgroups but the
  * scheduler and the reserve; *participating excludes the scheduler. */
 int wgm_serve_start(wgm_serve* s, int32_t* participating);

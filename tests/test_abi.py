@@ -134,3 +134,28 @@ __global__ void k(wgm_kernel_args wa, int* out) {
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "gpu_workgroup_managment_amd", "csrc")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_serve_plan_keeps_graphs_and_request_state_within_the_budget():
+    """The query server's request-state arena is planned on the host: it is
+    max_inflight x the largest request's state, cut to what the budget leaves
+    after the graphs, so the server never allocates more than its budget."""
+    from gpu_workgroup_managment_amd import graphs as G
+    from gpu_workgroup_managment_amd import serving as SV
+
+    gs = [G.add_weights(G.grid(40, 40), 1, 255, 3), G.rmat(10, 8), G.add_weights(G.rmat(11, 8), 1, 255, 3)]
+    gbytes, arena = SV.plan(gs, SV.config(max_inflight=32))
+    # 256-byte aligned row_ptr / col / wt per graph
+    al = lambda w: (w + 63) // 64 * 64 * 4  # noqa: E731
+    want_g = sum(al(g.n + 1) + al(g.m) + (al(g.m) if g.wt is not None else 0) for g in gs)
+    assert gbytes == want_g
+    nmax = max(g.n for g in gs)
+    largest = (7 * nmax + 4 + 63) // 64 * 64 * 4  # an SSSP request: 7n + 4 words
+    assert arena == 32 * largest  # a generous budget: every slot holds the largest request
+    for budget in (gbytes + 2 * largest, gbytes + largest + 1000, gbytes + 4096):
+        gb, ab = SV.plan(gs, SV.config(max_inflight=32, budget_bytes=budget))
+        assert gb == gbytes and gb + ab <= budget and ab >= 0
+        assert ab >= min(32 * largest, budget - gbytes - 255)
+    with pytest.raises(L.WgmError) as e:
+        SV.plan(gs, SV.config(budget_bytes=gbytes - 1))
+    assert e.value.code == -9

@@ -103,7 +103,11 @@ def test_serve_admission_budget(rt, pool):
     nmax = max(g.n for g in gs)
     state = ((4 * nmax + 2 + 63) // 64) * 64 * 4
     budget = gbytes + 2 * state  # room for two of the largest requests at a time
-    with S.Server(rt, gs, S.config(max_inflight=32, budget_bytes=budget)) as srv:
+    cfg = S.config(max_inflight=32, budget_bytes=budget)
+    with S.Server(rt, gs, cfg) as srv:
+        # the real allocation: graphs + request-state arena within the budget
+        assert (srv.graph_bytes, srv.arena_bytes) == S.plan(gs, cfg)
+        assert srv.graph_bytes + srv.arena_bytes <= budget
         srv.start()
         peak = 0
         for i in range(len(gs)):
@@ -127,7 +131,9 @@ def test_serve_admission_budget_sssp(rt, pool):
     nmax = max(g.n for g in gs)
     state = ((7 * nmax + 4 + 63) // 64) * 64 * 4
     budget = gbytes + 2 * state
-    with S.Server(rt, gs, S.config(max_inflight=32, budget_bytes=budget)) as srv:
+    cfg = S.config(max_inflight=32, budget_bytes=budget)
+    with S.Server(rt, gs, cfg) as srv:
+        assert srv.graph_bytes + srv.arena_bytes <= budget
         srv.start()
         peak = 0
         for i in range(len(gs)):
