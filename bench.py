@@ -43,7 +43,16 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--suite", choices=("batched", "apps"), default="batched",
                     help="apps: one line per standalone kernel on its workload (not the headline line)")
+    ap.add_argument("--far-flags-lds", type=int, default=None, choices=(0, 1),
+                    help="override the tuned setting of the LDS far-flag bitmap")
+    ap.add_argument("--only", choices=("rmat-bfs", "rmat-sssp", "grid-bfs", "grid-sssp"),
+                    help="time one class of the batched workload (not the headline line)")
+    ap.add_argument("--apps", default="", help="with --suite apps: a subset of bfs,sssp,bc,grid,barrier")
+    ap.add_argument("--shapes", default="", help="with --suite apps: launch shapes WGSIZExNUMWGS,... (0: occupancy)")
+    ap.add_argument("--reps", type=int, default=5, help="with --suite apps: timed launches per shape")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configured workloads (ER BFS, RMAT-20 SSSP, grid colour/MIS/PageRank, barrier)")
     ap.add_argument("--serving-s", type=float, default=0.5, help="arrival window per offered load")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run a plain copy kernel of BYTES (HBM counter calibration under rocprofv3)")
@@ -81,17 +90,34 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(graphs, queries, batch, budget_s):
-    """Threaded oracle (threads play workgroups, one barrier per level) on
-    the first graphs of the batch until the time budget is spent; returns the
-    baseline entry and whether every sampled GPU result matched."""
+def cpu_info():
+    """(usable cores, model name) of this host."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(graphs, queries, batch, budget_s, repeats=3):
+    """Threaded restatement (threads play workgroups, one barrier per level),
+    pinned one thread per core, on the first graphs of the batch that fit a
+    time budget; the sample is timed `repeats` times and the median GTEPS is
+    reported. Also returns whether every sampled GPU result matched.
+    Threads: the usable cores, at most 16 (one GPU's share of the host)."""
     import numpy as np
     import oracle
-    threads = min(16, os.cpu_count() or 1)
-    edges = 0
-    secs = 0.0
-    done = 0
-    exact = True
+    cores, model = cpu_info()
+    threads = max(1, min(16, cores))
+    oracle.threads_pin(True)
+    per = budget_s / repeats
+    sample, secs, edges, exact = [], 0.0, 0, True
     for i, (g, q) in enumerate(zip(graphs, queries)):
         t0 = time.perf_counter()
         if q == 0:
@@ -102,18 +128,31 @@ def cpu_baseline(graphs, queries, batch, budget_s):
         edges += int(g.degrees()[ref != oracle.INF].sum())
         dist, _, _ = batch.result(i)
         exact = exact and bool(np.array_equal(dist, ref))
-        done += 1
-        if secs >= budget_s:
+        sample.append(i)
+        if secs >= per:
             break
-    gteps = edges / secs / 1e9 if secs > 0 else 0.0
+    runs = [secs]
+    for _ in range(repeats - 1):
+        t0 = time.perf_counter()
+        for i in sample:
+            g = graphs[i]
+            if queries[i] == 0:
+                oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
+            else:
+                oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
+        runs.append(time.perf_counter() - t0)
+    oracle.threads_pin(False)
+    med = sorted(runs)[len(runs) // 2]
+    gteps = edges / med / 1e9 if med > 0 else 0.0
     return {
         "value": round(gteps, 5),
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {done} of {len(graphs)} batch queries ({edges} edges scanned), "
-                  f"threaded frontier BFS/Bellman-Ford restatement, {secs:.2f} s",
-    }, exact, done
+        "sample": f"first {len(sample)} of {len(graphs)} batch queries ({edges} edges scanned), threaded frontier "
+                  f"BFS/Bellman-Ford restatement, {threads} threads pinned one per core on {model} "
+                  f"({cores} usable cores), median of {repeats} runs of {min(runs):.2f}-{max(runs):.2f} s",
+    }, exact, len(sample)
 
 
 def serving_sweep(rt, graphs, queries, batch_digests, loads, duration_s, np_fraction=0.5, np_groups=16):
@@ -170,7 +209,113 @@ def serving_sweep(rt, graphs, queries, batch_digests, loads, duration_s, np_frac
     }
 
 
-def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5):
+def configs_leg(rt, reps=5):
+    """The other configured workloads on their standalone kernels, each on its
+    full input and checked against the oracle, for the one JSON line: BFS on
+    ER 2^16 (avg degree 8), SSSP on RMAT scale 20, colour / MIS / PageRank on
+    the 2048 x 2048 grid (launch shapes as measured best), and the barrier
+    round trip at full occupancy. kernel_ms is the median HIP-event time of
+    `reps` launches; the roofline uses each kernel's algorithmic bytes; the
+    CPU figure is the oracle on this host (threaded and pinned for BFS/SSSP,
+    serial otherwise)."""
+    import numpy as np
+
+    import oracle
+    from gpu_workgroup_managment_amd import graphs as G
+
+    cores, _ = cpu_info()
+    threads = max(1, min(16, cores))
+    out = {}
+
+    def timed(fn):
+        fn()  # warm-up (and the discovery cache)
+        runs = [fn() for _ in range(reps)]
+        k = sorted(r[1]["kernel_ms"] for r in runs)[reps // 2]
+        return k, runs[-1]
+
+    def entry(kernel_ms, alg_bytes, exact, cpu_s, extra):
+        ach = alg_bytes / (kernel_ms * 1e6)
+        e = {"kernel_ms": round(kernel_ms, 4), "bitexact": bool(exact), "algorithmic_bytes": int(alg_bytes),
+             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(ach / HBM_PEAK_GBS, 5)},
+             "cpu_oracle_ms": round(cpu_s * 1e3, 2)}
+        e.update(extra)
+        return e
+
+    oracle.threads_pin(True)
+    # 1. BFS, ER 2^16
+    g = G.er_config()
+    dg = rt.graph(g)
+    t0 = time.perf_counter()
+    ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
+    cpu = time.perf_counter() - t0
+    k, (res, st) = timed(lambda: dg.bfs(0, wg_size=1024, num_wgs=64))
+    reached = ref != oracle.INF
+    ms = int(g.degrees()[reached].sum())
+    out["er_bfs"] = entry(k, 8 * ms + 20 * int(reached.sum()), np.array_equal(res, ref), cpu, {
+        "workload": "ER n=2^16 avg degree 8, source 0", "levels": st["iterations"], "gteps": round(ms / (k * 1e6), 4),
+        "us_per_level": round(st["device_ms"] * 1e3 / max(1, st["iterations"]), 2),
+        "launch": {"wg_size": 1024, "num_wgs": 64}, "cpu_threads": threads})
+    dg.close()
+    # 2. SSSP, RMAT scale 20
+    g = G.rmat_config()
+    dg = rt.graph(g)
+    t0 = time.perf_counter()
+    ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
+    cpu = time.perf_counter() - t0
+    k, (res, st) = timed(lambda: dg.sssp(0, wg_size=1024, num_wgs=64))
+    reached = ref != oracle.INF
+    ms = int(g.degrees()[reached].sum())
+    out["rmat20_sssp"] = entry(k, 12 * ms + 20 * int(reached.sum()), np.array_equal(res, ref), cpu, {
+        "workload": "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", "levels": st["iterations"],
+        "gteps": round(ms / (k * 1e6), 4), "edges_relaxed": st["edges_scanned"],
+        "launch": {"wg_size": 1024, "num_wgs": 64}, "cpu_threads": threads})
+    dg.close()
+    oracle.threads_pin(False)
+    # 3. colour / MIS / PageRank, grid 2048^2
+    g = G.grid_config()
+    dg = rt.graph(g)
+    n, m = g.n, g.m
+    deg = g.degrees().astype(np.int64)
+    t0 = time.perf_counter()
+    cref, crounds = oracle.color(g.row_ptr, g.col, g.prio)
+    cpu = time.perf_counter() - t0
+    k, (res, st) = timed(lambda: dg.color(wg_size=1024, num_wgs=0))
+    # per round: read every colour + write max_d (8n); per vertex uncoloured at round r: row pair, neighbour
+    # colours and priorities (8 + 8 deg), its priority and max (8)
+    alg = crounds * 8 * n + int((cref.astype(np.int64) * (16 + 8 * deg)).sum())
+    out["grid_color"] = entry(k, alg, np.array_equal(res, cref), cpu, {
+        "workload": "grid 2048x2048, uint32 priorities seed 4", "rounds": st["iterations"], "cpu_threads": 1})
+    t0 = time.perf_counter()
+    sref, mrounds = oracle.mis(g.row_ptr, g.col, g.prio)
+    cpu = time.perf_counter() - t0
+    k, (res, st) = timed(lambda: dg.mis(wg_size=1024, num_wgs=0))
+    # lower bound: every round reads every vertex's state; the first reads every adjacency list
+    alg = mrounds * 4 * n + 8 * m + 4 * (n + 1)
+    out["grid_mis"] = entry(k, alg, np.array_equal(res, sref), cpu, {
+        "workload": "grid 2048x2048, uint32 priorities seed 4", "rounds": st["iterations"], "cpu_threads": 1})
+    t0 = time.perf_counter()
+    p64 = oracle.pagerank_f64(g.row_ptr, g.col, 0.85, 50)
+    cpu = time.perf_counter() - t0
+    p32 = oracle.pagerank_f32(g.row_ptr, g.col, np.float32(0.85), 50)
+    k, (res, st) = timed(lambda: dg.pagerank(0.85, 50, wg_size=1024, num_wgs=0))
+    out["grid_pagerank"] = entry(k, 50 * (4 * (n + 1) + 8 * m + 20 * n), np.array_equal(res.view(np.uint32), p32.view(np.uint32)),
+                                 cpu, {"workload": "grid 2048x2048, d=0.85, 50 iterations, FP32",
+                                       "l1_vs_fp64": float(np.abs(res.astype(np.float64) - p64).sum()),
+                                       "bitexact_vs": "FP32 oracle", "cpu_threads": 1})
+    dg.close()
+    # 4. barrier round trip at full occupancy (os_freq): 10 x iterations barriers per launch
+    bar = {}
+    for var, name in ((2, "xcd"), (0, "flags"), (1, "sense"), (3, "hip_cooperative_grid_sync")):
+        for wg in (1024, 256):
+            r = rt.barrier_bench(wg, 0, 200, var)
+            bar[f"{name}_wg{wg}"] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
+                                     "epoch_errors": r["epoch_errors"]}
+    out["barrier_full_occupancy"] = bar
+    return out
+
+
+def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5, apps=None):
     """Standalone persistent kernels on their configured workloads: BFS on
     ER 2^16 (avg degree 8), SSSP on RMAT scale 20 (ef 16, w in [1, 255]),
     color / MIS / PageRank on the 2048x2048 grid. One JSON line per app with
@@ -185,6 +330,9 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
     rt = wgm.Runtime(0)
     threads = min(16, os.cpu_count() or 1)
     out = []
+
+    def want(name):
+        return apps is None or name in apps
 
     def timed(fn):
         best = None
@@ -211,104 +359,109 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
         out.append(rec)
         print(json.dumps(rec), flush=True)
 
-    # BFS on ER
-    g = G.er_config()
-    dg = rt.graph(g)
-    t0 = time.perf_counter()
-    ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.bfs(0, **lk))
-    res = best[4][0]
-    reached = ref != oracle.INF
-    ms = int(g.degrees()[reached].sum())
-    line("bfs", "ER n=2^16 avg degree 8 (m=524288), source 0", best, 8 * ms + 20 * int(reached.sum()),
-         bool(np.array_equal(res, ref)), {"edges_scanned": ms, "gteps": round(ms / (best[0] * 1e6), 4),
-                                         "us_per_level": round(best[1] * 1e3 / max(1, best[4][1]["iterations"]), 2)},
-         cpu, f"threaded frontier BFS, {threads} threads")
-    dg.close()
-    # SSSP on RMAT 20
-    g = G.rmat_config()
-    dg = rt.graph(g)
-    t0 = time.perf_counter()
-    ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.sssp(0, **lk))
-    res = best[4][0]
-    reached = ref != oracle.INF
-    ms = int(g.degrees()[reached].sum())
-    line("sssp", "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", best,
-         12 * ms + 20 * int(reached.sum()), bool(np.array_equal(res, ref)),
-         {"edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
-          "relaxations_scanned": best[4][1]["edges_scanned"]},
-         cpu, f"threaded frontier Bellman-Ford, {threads} threads")
-    # Pannotia SSSP (min-plus sweeps over in-edges) on the same graph
-    t0 = time.perf_counter()
-    mref, jacobi = oracle.sssp_minplus(g.row_ptr, g.col, g.wt, 0)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.sssp_minplus(0, **lk))
-    sweeps = best[4][1]["iterations"]
-    indeg = np.bincount(g.col, minlength=g.n)
-    lanes = 4 if g.m < 8 * g.n else 16 if g.m < 32 * g.n else 64
-    ntasks = int(((indeg + 16 * lanes - 1) // (16 * lanes)).sum())
-    # per sweep: the task list, source + weight of every in-edge, one dist gather per in-edge
-    line("sssp_minplus", "Pannotia SSSP, RMAT scale 20 ef 16, w in [1,255], source 0", best,
-         sweeps * (16 * ntasks + 12 * g.m), bool(np.array_equal(best[4][0], mref)),
-         {"sweeps": sweeps, "jacobi_sweeps_oracle": jacobi, "tasks": ntasks,
-          "gteps_sweep_edges": round(sweeps * g.m / (best[0] * 1e6), 4)},
-         cpu, "serial Jacobi restatement")
-    dg.close()
-    # Pannotia BC, all sources
-    g = G.rmat(12, 8)
-    dg = rt.graph(g)
-    t0 = time.perf_counter()
-    bref, blevels = oracle.bc(g.row_ptr, g.col)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.bc(0, **lk))
-    # per source: out-edges twice (BFS 8 B, dependency pull 16 B), in-edges once (8 B), 28 B per vertex
-    line("bc", "Pannotia BC, RMAT scale 12 ef 8, all 4096 sources, FP64", best,
-         g.n * (24 * g.m + 8 * g.m + 28 * g.n), bool(np.array_equal(best[4][0], bref)),
-         {"levels": blevels, "gteps": round(g.n * g.m / (best[0] * 1e6), 4)}, cpu, "serial FP64 Brandes")
-    dg.close()
-    # color / MIS / PageRank on the grid
-    g = G.grid_config()
-    dg = rt.graph(g)
-    n, m = g.n, g.m
-    deg = g.degrees().astype(np.int64)
-    t0 = time.perf_counter()
-    cref, crounds = oracle.color(g.row_ptr, g.col, g.prio)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.color(**lk))
-    # per round: read every color + write max_d (8n); per uncolored vertex: row pair, neighbour
-    # colors and priorities (8 + 8 deg), its priority and max (8)
-    alg = crounds * 8 * n + int((cref.astype(np.int64) * (16 + 8 * deg)).sum())
-    line("color", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], cref)),
-         {"rounds_oracle": crounds}, cpu, "serial restatement")
-    t0 = time.perf_counter()
-    sref, mrounds = oracle.mis(g.row_ptr, g.col, g.prio)
-    cpu = time.perf_counter() - t0
-    best = timed(lambda **lk: dg.mis(**lk))
-    # lower bound: every round reads the state of every vertex; the first reads every adjacency list
-    alg = mrounds * 4 * n + 8 * m + 4 * (n + 1)
-    line("mis", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], sref)),
-         {"rounds_oracle": mrounds}, cpu, "serial restatement")
-    t0 = time.perf_counter()
-    p64 = oracle.pagerank_f64(g.row_ptr, g.col, 0.85, 50)
-    cpu = time.perf_counter() - t0
-    p32 = oracle.pagerank_f32(g.row_ptr, g.col, np.float32(0.85), 50)
-    best = timed(lambda **lk: dg.pagerank(0.85, 50, **lk))
-    pr = best[4][0]
-    alg = 50 * (4 * (n + 1) + 8 * m + 20 * n)
-    line("pagerank", "grid 2048x2048, d=0.85, 50 iterations, FP32", best, alg, bool(np.array_equal(pr, p32)),
-         {"l1_vs_fp64": float(np.abs(pr.astype(np.float64) - p64).sum()), "bitexact_vs": "FP32 oracle"},
-         cpu, "serial FP64")
-    dg.close()
-    # barrier round trip at full occupancy
-    for var, name in ((0, "flags"), (2, "xcd"), (1, "sense")):
-        for wg in (1024, 256):
-            r = rt.barrier_bench(wg, 0, 200, var)
-            rec = {"app": "barrier", "variant": name, "wg_size": wg, "participating": r["participating"],
-                   "ns_per_barrier": round(r["ns_per_barrier"], 1), "epoch_errors": r["epoch_errors"]}
-            print(json.dumps(rec), flush=True)
+    if want("bfs"):
+        # BFS on ER
+        g = G.er_config()
+        dg = rt.graph(g)
+        t0 = time.perf_counter()
+        ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.bfs(0, **lk))
+        res = best[4][0]
+        reached = ref != oracle.INF
+        ms = int(g.degrees()[reached].sum())
+        line("bfs", "ER n=2^16 avg degree 8 (m=524288), source 0", best, 8 * ms + 20 * int(reached.sum()),
+             bool(np.array_equal(res, ref)), {"edges_scanned": ms, "gteps": round(ms / (best[0] * 1e6), 4),
+                                             "us_per_level": round(best[1] * 1e3 / max(1, best[4][1]["iterations"]), 2)},
+             cpu, f"threaded frontier BFS, {threads} threads")
+        dg.close()
+    if want("sssp"):
+        # SSSP on RMAT 20
+        g = G.rmat_config()
+        dg = rt.graph(g)
+        t0 = time.perf_counter()
+        ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.sssp(0, **lk))
+        res = best[4][0]
+        reached = ref != oracle.INF
+        ms = int(g.degrees()[reached].sum())
+        line("sssp", "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", best,
+             12 * ms + 20 * int(reached.sum()), bool(np.array_equal(res, ref)),
+             {"edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
+              "relaxations_scanned": best[4][1]["edges_scanned"]},
+             cpu, f"threaded frontier Bellman-Ford, {threads} threads")
+        # Pannotia SSSP (min-plus sweeps over in-edges) on the same graph
+        t0 = time.perf_counter()
+        mref, jacobi = oracle.sssp_minplus(g.row_ptr, g.col, g.wt, 0)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.sssp_minplus(0, **lk))
+        sweeps = best[4][1]["iterations"]
+        indeg = np.bincount(g.col, minlength=g.n)
+        lanes = 4 if g.m < 8 * g.n else 16 if g.m < 32 * g.n else 64
+        ntasks = int(((indeg + 16 * lanes - 1) // (16 * lanes)).sum())
+        # per sweep: the task list, source + weight of every in-edge, one dist gather per in-edge
+        line("sssp_minplus", "Pannotia SSSP, RMAT scale 20 ef 16, w in [1,255], source 0", best,
+             sweeps * (16 * ntasks + 12 * g.m), bool(np.array_equal(best[4][0], mref)),
+             {"sweeps": sweeps, "jacobi_sweeps_oracle": jacobi, "tasks": ntasks,
+              "gteps_sweep_edges": round(sweeps * g.m / (best[0] * 1e6), 4)},
+             cpu, "serial Jacobi restatement")
+        dg.close()
+    if want("bc"):
+        # Pannotia BC, all sources
+        g = G.rmat(12, 8)
+        dg = rt.graph(g)
+        t0 = time.perf_counter()
+        bref, blevels = oracle.bc(g.row_ptr, g.col)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.bc(0, **lk))
+        # per source: out-edges twice (BFS 8 B, dependency pull 16 B), in-edges once (8 B), 28 B per vertex
+        line("bc", "Pannotia BC, RMAT scale 12 ef 8, all 4096 sources, FP64", best,
+             g.n * (24 * g.m + 8 * g.m + 28 * g.n), bool(np.array_equal(best[4][0], bref)),
+             {"levels": blevels, "gteps": round(g.n * g.m / (best[0] * 1e6), 4)}, cpu, "serial FP64 Brandes")
+        dg.close()
+    if want("grid"):
+        # color / MIS / PageRank on the grid
+        g = G.grid_config()
+        dg = rt.graph(g)
+        n, m = g.n, g.m
+        deg = g.degrees().astype(np.int64)
+        t0 = time.perf_counter()
+        cref, crounds = oracle.color(g.row_ptr, g.col, g.prio)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.color(**lk))
+        # per round: read every color + write max_d (8n); per uncolored vertex: row pair, neighbour
+        # colors and priorities (8 + 8 deg), its priority and max (8)
+        alg = crounds * 8 * n + int((cref.astype(np.int64) * (16 + 8 * deg)).sum())
+        line("color", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], cref)),
+             {"rounds_oracle": crounds}, cpu, "serial restatement")
+        t0 = time.perf_counter()
+        sref, mrounds = oracle.mis(g.row_ptr, g.col, g.prio)
+        cpu = time.perf_counter() - t0
+        best = timed(lambda **lk: dg.mis(**lk))
+        # lower bound: every round reads the state of every vertex; the first reads every adjacency list
+        alg = mrounds * 4 * n + 8 * m + 4 * (n + 1)
+        line("mis", "grid 2048x2048, uint32 priorities seed 4", best, alg, bool(np.array_equal(best[4][0], sref)),
+             {"rounds_oracle": mrounds}, cpu, "serial restatement")
+        t0 = time.perf_counter()
+        p64 = oracle.pagerank_f64(g.row_ptr, g.col, 0.85, 50)
+        cpu = time.perf_counter() - t0
+        p32 = oracle.pagerank_f32(g.row_ptr, g.col, np.float32(0.85), 50)
+        best = timed(lambda **lk: dg.pagerank(0.85, 50, **lk))
+        pr = best[4][0]
+        alg = 50 * (4 * (n + 1) + 8 * m + 20 * n)
+        line("pagerank", "grid 2048x2048, d=0.85, 50 iterations, FP32", best, alg, bool(np.array_equal(pr, p32)),
+             {"l1_vs_fp64": float(np.abs(pr.astype(np.float64) - p64).sum()), "bitexact_vs": "FP32 oracle"},
+             cpu, "serial FP64")
+        dg.close()
+    if want("barrier"):
+        # barrier round trip at full occupancy
+        for var, name in ((0, "flags"), (2, "xcd"), (1, "sense"), (3, "hip_cooperative_grid_sync")):
+            for wg in (1024, 256):
+                r = rt.barrier_bench(wg, 0, 200, var)
+                rec = {"app": "barrier", "variant": name, "wg_size": wg, "participating": r["participating"],
+                       "ns_per_barrier": round(r["ns_per_barrier"], 1), "epoch_errors": r["epoch_errors"]}
+                print(json.dumps(rec), flush=True)
     rt.close()
     return out
 
@@ -316,7 +469,12 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
 def main():
     args = parse()
     if args.suite == "apps":
-        suite_apps()
+        shapes = tuple(tuple(int(x) for x in sh.split("x")) for sh in args.shapes.split(",")) if args.shapes else None
+        apps = set(args.apps.split(",")) if args.apps else None
+        if shapes:
+            suite_apps(shapes=shapes, reps=args.reps, apps=apps)
+        else:
+            suite_apps(reps=args.reps, apps=apps)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -329,6 +487,12 @@ def main():
 
     t0 = time.time()
     items, graphs, queries = B.workload(args.count, threads=min(16, os.cpu_count() or 1))
+    if args.only:
+        # one class of the workload (profiling runs that attribute traffic per class)
+        kind = {"rmat": 0, "grid": 1}[args.only.split("-")[0]]
+        query = {"bfs": 0, "sssp": 1}[args.only.split("-")[1]]
+        keep = [i for i, it in enumerate(items) if it.kind == kind and it.query == query]
+        graphs, queries = [graphs[i] for i in keep], queries[keep]
     gen_s = time.time() - t0
     rt = wgm.Runtime(0)
     copy_cal = None
@@ -340,6 +504,8 @@ def main():
     if args.autotune:
         best, _ = batch.autotune(1)
         cfg_d = {k: v for k, v in best.as_dict().items() if k not in ("use_graph", "reserved")}
+    if args.far_flags_lds is not None:
+        cfg_d["far_flags_lds"] = args.far_flags_lds
     cfg = B.config(use_graph=True, **cfg_d)
 
     for _ in range(args.warmup):
@@ -367,7 +533,15 @@ def main():
             r["shape"] = {k: shape.get(k, 0) for k in ("team_edges", "max_team", "delta", "delta_high_degree")}
             relaunch_tried.append({"gteps": round(r["gteps"], 4), **r["shape"]})
             relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
-    # the relaunch overwrote results: run the batched launch once more for the checks below
+    # baseline 2: each query on the standalone app kernel (wgm_bfs / wgm_sssp), one launch per graph
+    standalone = None
+    for wg in (1024, 256):
+        r = batch.run_standalone(wg_size=wg)
+        relaunch_tried.append({"gteps": round(r["gteps"], 4), "kernel": "standalone wgm_bfs/wgm_sssp", "wg_size": wg})
+        r["shape"] = {"kernel": "standalone wgm_bfs/wgm_sssp", "wg_size": wg}
+        standalone = r if standalone is None or r["wall_ms"] < standalone["wall_ms"] else standalone
+        relaunch = r if r["wall_ms"] < relaunch["wall_ms"] else relaunch
+    # the baselines overwrote results: run the batched launch once more for the checks below
     batch.run(cfg)
 
     dists = [batch.result(i)[0] for i in range(len(graphs))]
@@ -394,6 +568,9 @@ def main():
         batch.close()
         serving = serving_sweep(rt, graphs, queries, digests, (0.25, 0.5, 0.75), args.serving_s)
         p99_at_half = next(x["task_p99_us"] for x in serving["loads"] if x["offered_load"] == 0.5)
+    else:
+        batch.close()
+    configs = None if args.no_configs else configs_leg(rt)
     line = {
         "metric": "batched-graph GTEPS & p99 task latency vs per-graph relaunch, 1 MI355X, bit-exact",
         "value": round(gteps, 4),
@@ -410,7 +587,8 @@ def main():
         "config": {
             "workload": f"batched-{args.count}: n log-uniform [2^10, 2^18], even RMAT ef16 truncated, odd grids, "
                         f"w in [1,255], BFS+SSSP mixed, one persistent launch (hipGraph)",
-            "graphs": args.count,
+            "graphs": len(graphs),
+            "only": args.only,
             "vertices": int(sum(g.n for g in graphs)),
             "edges": int(sum(g.m for g in graphs)),
             "edges_scanned_per_step": int(edges),
@@ -426,6 +604,7 @@ def main():
             "shape": relaunch["shape"],
             "tried": relaunch_tried,
             "speedup_batched_vs_relaunch": round(gteps / relaunch["gteps"], 2) if relaunch["gteps"] else None,
+            "standalone_kernel_gteps": round(standalone["gteps"], 4),
         },
         "copy_calibration": copy_cal,
         "bitexact_sampled": exact,
@@ -436,11 +615,11 @@ def main():
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": base,
+        "configs": configs,
         "device": info["arch"],
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line))
-    batch.close()
     rt.close()
 
 

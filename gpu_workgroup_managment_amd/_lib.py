@@ -39,7 +39,7 @@ SOL_COLON, SOL_COLON_SPACE, SOL_PLAIN = range(3)
 BIG_NUM = 99999999  # WGM_BIG_NUM: Pannotia SSSP's unreached distance
 GRAPH_RMAT, GRAPH_GRID = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
-BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD = 0, 1, 2
+BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD, BARRIER_COOP = 0, 1, 2, 3
 
 
 class WgmError(RuntimeError):
@@ -130,6 +130,7 @@ class BatchConfig(ctypes.Structure):
         ("delta", ctypes.c_int32),
         ("sssp_team_edges", ctypes.c_int32),
         ("delta_high_degree", ctypes.c_int32),
+        ("far_flags_lds", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -209,7 +210,48 @@ class ServeReport(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class PersistentTask(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("src", ctypes.c_int32), ("damping", ctypes.c_float),
+                ("iterations", ctypes.c_int32)]
+
+
+class PersistentApp(ctypes.Structure):
+    """wgm_persistent_app: the plugin's state and its nine entry points
+    (function pointers, filled by the plugin's own C code)."""
+    _fields_ = [("self", ctypes.c_void_p)] + [(f, ctypes.c_void_p) for f in (
+        "name", "kernel", "init_for_occupancy", "init_for_real", "set_args_for_occupancy", "set_args_for_real",
+        "reset", "check", "clean")]
+
+
+class NpTask(ctypes.Structure):
+    """wgm_np_task: the non-persistent task plugin's state and entry points."""
+    _fields_ = [("self", ctypes.c_void_p)] + [(f, ctypes.c_void_p) for f in (
+        "name", "kernel", "init", "set_args", "reset", "check", "clean")]
+
+
+class SkelConfig(ctypes.Structure):
+    _fields_ = [("threads_per_wg", ctypes.c_int32), ("num_wgs", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("non_persistent_wgs", ctypes.c_int32), ("non_persistent_frequency_ms", ctypes.c_double),
+                ("skip_tasks", ctypes.c_int32), ("use_query_barrier", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
+                ("max_tasks", ctypes.c_int32)]
+
+
+class SkelReport(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int32) for f in ("occupancy", "num_wgs", "np_wgs", "iterations", "persistent_errors",
+                                              "np_errors", "tasks", "redlines")] + [
+        (f, ctypes.c_double) for f in ("total_ms", "mean_ms", "min_ms", "max_ms", "std_ms", "response_p50_us",
+                                       "response_p99_us", "exec_p50_us", "exec_p99_us")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 EXTRA_STRUCTS = {
+    "wgm_persistent_app": PersistentApp,
+    "wgm_np_task": NpTask,
+    "wgm_skel_config": SkelConfig,
+    "wgm_skel_report": SkelReport,
+    "wgm_persistent_task": PersistentTask,
     "wgm_serve_config": ServeConfig,
     "wgm_serve_done": ServeDone,
     "wgm_serve_load": ServeLoad,
@@ -264,6 +306,7 @@ SIGNATURES = {
     "wgm_mega_destroy": (_i32, [_vp]),
     "wgm_mega_launch": (_i32, [_vp, _i32, _i32, _i32, _P(_i32)]),
     "wgm_mega_send_persistent": (_i32, [_vp, _vp, _i32, _i32, _i32]),
+    "wgm_mega_send_persistent_task": (_i32, [_vp, _vp, _P(PersistentTask), _i32]),
     "wgm_mega_persistent_running": (_i32, [_vp]),
     "wgm_mega_wait_persistent": (_i32, [_vp, ctypes.c_double, _P(ctypes.c_double)]),
     "wgm_mega_send_task": (_i32, [_vp, _i32, _P(TaskTimes)]),
@@ -282,6 +325,10 @@ SIGNATURES = {
     "wgm_mega_launch_custom": (_i32, [_vp, _vp, _i32, _i32, _i32, _vp, _P(_i32)]),
     "wgm_mega_send_persistent_custom": (_i32, [_vp, _i32]),
     "wgm_mega_send_task_custom": (_i32, [_vp, _i32, _P(TaskTimes)]),
+    "wgm_discover_occupancy_custom": (_i32, [_vp, _vp, _i32, _i32, _vp, _i32, _P(_i32)]),
+    "wgm_mega_set_query_barrier": (_i32, [_vp, _i32]),
+    "wgm_run_persistent": (_i32, [_vp, _P(PersistentApp), _P(SkelConfig), _P(SkelReport)]),
+    "wgm_run_merged": (_i32, [_vp, _P(PersistentApp), _P(NpTask), _vp, _P(SkelConfig), _P(SkelReport)]),
     "wgm_serve_create": (_i32, [_vp, _i32, _P(Csr), _P(ServeConfig), _P(_vp)]),
     "wgm_serve_destroy": (_i32, [_vp]),
     "wgm_serve_graph_bytes": (_i64, [_vp]),
@@ -301,6 +348,7 @@ SIGNATURES = {
     "wgm_batch_bytes": (_i64, [_vp]),
     "wgm_batch_run": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
     "wgm_batch_run_relaunch": (_i32, [_vp, _vp, _P(BatchConfig), _P(BatchStats)]),
+    "wgm_batch_run_standalone": (_i32, [_vp, _vp, _P(Launch), _P(BatchStats)]),
     "wgm_batch_result": (_i32, [_vp, _i32, _vp, _P(_i32), _P(_i64)]),
     "wgm_batch_job_times": (_i32, [_vp, _vp, _vp]),
     "wgm_batch_autotune": (_i32, [_vp, _vp, _i32, _P(BatchConfig), _P(BatchStats)]),

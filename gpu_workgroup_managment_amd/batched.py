@@ -10,11 +10,12 @@ from . import graphs as G
 
 
 def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False, delta=0,
-           sssp_team_edges=0, delta_high_degree=0):
+           sssp_team_edges=0, delta_high_degree=0, far_flags_lds=0):
     c = L.BatchConfig()
     c.wg_size, c.wgs_per_cu, c.chunk = wg_size, wgs_per_cu, chunk
     c.team_edges, c.max_team, c.use_graph = team_edges, max_team, 1 if use_graph else 0
     c.delta, c.sssp_team_edges, c.delta_high_degree = delta, sssp_team_edges, delta_high_degree
+    c.far_flags_lds = 1 if far_flags_lds else 0
     return c
 
 
@@ -55,6 +56,14 @@ class Batch:
         cfg = cfg if cfg is not None else config(**kw)
         st = L.BatchStats()
         self.rt._check(L.lib().wgm_batch_run_relaunch(self.rt.handle, self._h, ctypes.byref(cfg), ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_standalone(self, wg_size=1024, num_wgs=0):
+        """Baseline: each query on wgm_bfs / wgm_sssp's standalone kernel, one launch per graph."""
+        lc = L.Launch()
+        lc.wg_size, lc.num_wgs = wg_size, num_wgs
+        st = L.BatchStats()
+        self.rt._check(L.lib().wgm_batch_run_standalone(self.rt.handle, self._h, ctypes.byref(lc), ctypes.byref(st)))
         return st.as_dict()
 
     def result(self, i):

@@ -199,6 +199,17 @@ int wgm_graph_frontier_state(wgm_graph* g, bool sssp, FrontierDev* s) {
   return WGM_OK;
 }
 
+int wgm_graph_app_state(wgm_graph* g, int app) {
+  wgm_runtime* rt = g->rt;
+  int rc = WGM_OK;
+  if (app == 1 || app == 2) rc = dev_alloc(rt, &g->i32a, g->n);
+  if (!rc && app == 2) rc = dev_alloc(rt, &g->i32b, g->n);
+  if (!rc && app == 3) rc = dev_alloc(rt, &g->f0, g->n);
+  if (!rc && app == 3) rc = dev_alloc(rt, &g->f1, g->n);
+  if (!rc && app == 3) rc = dev_alloc(rt, &g->f2, g->n);
+  return rc;
+}
+
 // State of the standalone team traversal: queues of max(n, m) + 1 entries
 // with distance tags for SSSP (a level queues at most one entry per
 // relaxation), n + 1 for BFS; claim counters; near-far queues and flags.
@@ -245,7 +256,7 @@ static int team_frontier_state(wgm_graph* g, bool sssp, FrontierDev* s, NearFar*
     nf->q[1] = g->far + g->n + 1;
     nf->flag = g->in_far;
     nf->cnt = g->fcnt;
-    nf->delta = g->maxdeg > 4 ? kStandaloneDeltaHighDegree : kStandaloneDelta;
+    nf->delta = wgm_standalone_delta(g->maxdeg);
   }
   return WGM_OK;
 }
@@ -363,6 +374,60 @@ int64_t wgm_graph_bytes(const wgm_graph* g) {
   return b;
 }
 
+}  // extern "C"
+
+// The standalone frontier kernel on any state: prologue (grid, control
+// reset, start event), launch on `grid` workgroups of `wg` threads, then the
+// finish (HIP event time, device clock, participants); *iters_dev is the
+// device cell the kernel writes its level count to. Allocates nothing.
+uint32_t wgm_standalone_delta(int maxdeg) { return maxdeg > 4 ? kStandaloneDeltaHighDegree : kStandaloneDelta; }
+
+// Enqueue the standalone frontier kernel (no wait): chunked claims of one
+// workgroup's width, heavy entries above heavy_threshold(wg).
+int wgm_enqueue_frontier(wgm_runtime* rt, const GraphDev& g, FrontierDev s, const NearFar& nf, bool sssp, int src,
+                         int grid, int wg, int* iters_dev) {
+  s.chunk = wg;  // frontier entries a workgroup claims at a time
+  s.heavy = heavy_threshold(wg);
+  const size_t lds = frontier_lds_bytes(wg);
+  if (sssp)
+    hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, g, s, nf, src, iters_dev);
+  else
+    hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, g, s, nf, src, iters_dev);
+  HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
+int wgm_launch_frontier(wgm_runtime* rt, const GraphDev& g, FrontierDev s, const NearFar& nf, bool sssp, int src,
+                        int grid, int wg, int handshake, int* iters_dev, wgm_run_stats* st) {
+  int rc = wgm_reset_ctl(rt, 0, handshake ? 1 : 0);
+  if (rc) return rc;
+  HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+  rc = wgm_enqueue_frontier(rt, g, s, nf, sssp, src, grid, wg, iters_dev);
+  if (rc) return rc;
+  double host_ms = 0.0;
+  if (handshake) rc = wgm_host_handshake(rt, &host_ms);
+  wgm_run_stats local{};
+  const int rc2 = wgm_finish_launch(rt, &local);
+  if (rc == WGM_OK) rc = rc2;
+  if (rc) return rc;
+  int iters = 0;
+  HIPCHK(rt, hipMemcpy(&iters, iters_dev, sizeof(int), hipMemcpyDeviceToHost));
+  local.iterations = iters;
+  local.host_ms = host_ms;
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+// Sum of out-degrees of the vertices with dist != INF into *out (a device
+// cell the caller has cleared), on the runtime's stream.
+int wgm_enqueue_reached_edges(wgm_runtime* rt, const GraphDev& g, const uint32_t* dist, unsigned long long* out) {
+  hipLaunchKernelGGL(k_reached_edges, dim3(256), dim3(256), 0, rt->stream, g, dist, out);
+  HIPCHK(rt, hipGetLastError());
+  return WGM_OK;
+}
+
+extern "C" {
+
 static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, uint32_t* dist_out,
                         wgm_run_stats* st, bool sssp) {
   if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
@@ -373,20 +438,13 @@ static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_la
   int rc = team_frontier_state(g, sssp, &s, &nf);
   if (rc) return rc;
   const int kernel = sssp ? WGM_K_SSSP : WGM_K_BFS;
-  int grid = 0, wg = 0;
-  rc = launch_prologue(rt, kernel, lc, &grid, &wg);
+  const int wg = lc && lc->wg_size > 0 ? lc->wg_size : 256;
+  int grid = 0;
+  rc = wgm_resolve_grid(rt, kernel, wg, lc ? lc->num_wgs : 0, &grid);
   if (rc) return rc;
-  s.chunk = wg;  // frontier entries a workgroup claims at a time
-  s.heavy = heavy_threshold(wg);
-  const size_t lds = frontier_lds_bytes(wg);
-  if (sssp)
-    hipLaunchKernelGGL(k_frontier<true>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, nf,
-                       src, g->iters);
-  else
-    hipLaunchKernelGGL(k_frontier<false>, dim3(grid), dim3(wg), lds, rt->stream, rt->ctl, wgm_graph_view(g), s, nf,
-                       src, g->iters);
   wgm_run_stats local{};
-  rc = launch_epilogue(rt, lc, g, &local);
+  rc = wgm_launch_frontier(rt, wgm_graph_view(g), s, nf, sssp, src, grid, wg, lc ? lc->handshake : 0, g->iters,
+                           &local);
   if (rc) return rc;
   rc = wgm_graph_reached_edges(g, &local.edges_scanned);
   if (rc) return rc;

@@ -60,4 +60,13 @@ bool wgm_pannotia_kernel_info(int kernel, KernelInfo* out);
 int wgm_pannotia_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
 void wgm_pannotia_release(wgm_graph* g);
 int wgm_graph_frontier_state(wgm_graph* g, bool sssp, wgm::FrontierDev* s);
+// per-run arrays of a topology-driven app: 1 colour (i32a), 2 MIS (i32a,
+// i32b), 3 PageRank (f0, f1, f2); 0 nothing
+int wgm_graph_app_state(wgm_graph* g, int app);
 int wgm_graph_reached_edges(wgm_graph* g, int64_t* out);
+int wgm_launch_frontier(wgm_runtime* rt, const wgm::GraphDev& g, wgm::FrontierDev s, const wgm::NearFar& nf, bool sssp,
+                        int src, int grid, int wg, int handshake, int* iters_dev, wgm_run_stats* st);
+int wgm_enqueue_frontier(wgm_runtime* rt, const wgm::GraphDev& g, wgm::FrontierDev s, const wgm::NearFar& nf, bool sssp,
+                         int src, int grid, int wg, int* iters_dev);
+uint32_t wgm_standalone_delta(int maxdeg);
+int wgm_enqueue_reached_edges(wgm_runtime* rt, const wgm::GraphDev& g, const uint32_t* dist, unsigned long long* out);

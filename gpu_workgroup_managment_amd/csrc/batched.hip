@@ -48,8 +48,14 @@ constexpr int kLowSlots = 1;
 // waiting on queue stores to reach L2. 2 * bd entries keep a 1024-thread
 // workgroup under 64 KB of LDS, and two such workgroups fit one CU.
 __host__ __device__ constexpr int low_lds_cap(int bd) { return 2 * bd; }
+// A 1024-thread workgroup is alone on its CU (its register and wave counts
+// allow one), so it may also hold the far flags of a low-degree SSSP job
+// as an LDS bitmap of kFarBitWords words (graphs up to 2^18 vertices):
+// setting a far flag is then an LDS atomic instead of a round trip to L2.
+constexpr int kFarBitWords = 8192;
+__host__ __device__ constexpr int batch_far_bits_bytes(int bd) { return bd >= 1024 ? 4 * kFarBitWords : 0; }
 __host__ __device__ constexpr int batch_lds_bytes(int bd) {
-  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd);
+  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd);
 }
 
 struct BatchJob {
@@ -90,6 +96,7 @@ struct BatchDev {
   unsigned long long* reached;  // per job: out-degrees of the reached vertices, summed after the traversal
   unsigned long long* relaxed;  // per job: edges relaxed from live frontier entries (work)
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
+  int far_bits;                 // 1: low-degree single-workgroup SSSP keeps its far flags in the LDS bitmap
 };
 
 
@@ -240,15 +247,21 @@ struct LowQueues {
 // band minimum.
 template <bool SSSP, int K, int U>
 __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
-                        int* iters_out) {
+                        int* iters_out, bool lds_flags) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
   const LowQueues lq{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
                      low_lds_cap((int)blockDim.x), &s};
+  // far flags: one bit per vertex in LDS (lds_flags), else a word per vertex in HBM
+  lds_u32* fbits = (lds_u32*)(reinterpret_cast<uint32_t*>(lds) + (frontier_lds_bytes((int)blockDim.x) +
+                                                                   2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
+  const bool bits = near_far && lds_flags;
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
-    if (near_far) nf.flag[v] = 0u;
+    if (near_far && !bits) nf.flag[v] = 0u;
   }
+  if (bits)
+    for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x) fbits[w] = 0u;
   if (threadIdx.x == 0) {
     lq.store(0, 0, src, row_info(g, src), 0u, SSSP);
     s_cnt[0] = 1;
@@ -365,8 +378,15 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP);
           }
           if (SSSP) {
-            const bool far_want = far_cand[x] && __hip_atomic_exchange(&nf.flag[u[x]], 1u, __ATOMIC_RELAXED,
-                                                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+            bool far_want;
+            if (bits) {
+              const uint32_t bit = 1u << (u[x] & 31);
+              far_want = far_cand[x] && (__hip_atomic_fetch_or(&fbits[u[x] >> 5], bit, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP) & bit) == 0u;
+            } else {
+              far_want = far_cand[x] && __hip_atomic_exchange(&nf.flag[u[x]], 1u, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+            }
             push_item<false>(far_want, u[x], nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
@@ -385,7 +405,10 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
       bool to_near = false, to_far = false;
       if (in) {
         if (dv < thr) {
-          nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
+          // each deferred vertex is in one far queue once: it moves once
+          if (bits) __hip_atomic_fetch_and(&fbits[v >> 5], ~(1u << (v & 31)), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+          else nf.flag[v] = 0u;
           to_near = dv >= t_old;
         } else {
           to_far = true;  // stays flagged in the other far queue
@@ -525,8 +548,9 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       // degree-bounded step for graphs whose out-degree never exceeds 4
       // (row starts must fit row_info's 29 bits)
       if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
-        if (sssp) run_low<true, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
-        else run_low<false, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+        const bool lds_flags = b.far_bits && batch_far_bits_bytes((int)blockDim.x) >= 4 * ((jb.n + 31) / 32);
+        if (sssp) run_low<true, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags);
+        else run_low<false, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false);
       } else {
         if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
         else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
@@ -665,6 +689,9 @@ struct wgm_batch {
   hipGraphExec_t exec = nullptr;
   wgm_batch_config graph_cfg{};
   int graph_grid = 0;
+  // standalone baseline: heavy lists for the largest graph and their 3 count lines
+  int32_t* sa_heavy = nullptr;
+  size_t sa_hcap = 0;
 };
 
 namespace {
@@ -806,6 +833,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   int rc = wgm_enqueue_reset_ctl(rt, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
+  d.far_bits = cfg.far_flags_lds ? 1 : 0;
   hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
   return enqueue_collect(b, st);
 }
@@ -991,7 +1019,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->res, b->dist, b->qa, b->qr,
-                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt};
+                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->sa_heavy};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -1078,6 +1106,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
   HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
   for (int k = 0; k < b->count; k++) {
     BatchDev one = d;
+    one.far_bits = cfg.far_flags_lds ? 1 : 0;
     one.jobs = b->d_jobs_one + k;  // same job, tickets numbered from 0
     one.njobs = 1;
     one.total_tickets = b->jobs[k].team;
@@ -1101,6 +1130,83 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
   wgm_batch_stats local{};
   local.launches = b->count;
   local.grid = grid;
+  rc = collect_results(b, &local);
+  if (rc) return rc;
+  local.device_ms = 0.0;  // per-launch clocks: only the wall time spans the whole baseline
+  local.participating = 0;
+  local.gteps = local.wall_ms > 0 ? (double)local.edges_scanned / (local.wall_ms * 1e6) : 0.0;
+  if (st) *st = local;
+  return WGM_OK;
+}
+
+// Baseline: one launch per query of the standalone app kernel of wgm_bfs /
+// wgm_sssp (every discovered workgroup as one team, its near-far bands and
+// heavy-entry pass), back to back on the stream, on the batch's resident
+// arrays, results checked like the batched launch's.
+int wgm_batch_run_standalone(wgm_runtime* rt, wgm_batch* b, const wgm_launch* lc, wgm_batch_stats* st) {
+  if (!rt || !b || b->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/batch");
+  const int wg = lc && lc->wg_size > 0 ? lc->wg_size : 1024;
+  int grid_bfs = 0, grid_sssp = 0;
+  int rc = wgm_resolve_grid(rt, WGM_K_BFS, wg, lc ? lc->num_wgs : 0, &grid_bfs);
+  if (!rc && b->weighted) rc = wgm_resolve_grid(rt, WGM_K_SSSP, wg, lc ? lc->num_wgs : 0, &grid_sssp);
+  if (rc) return rc;
+  if (b->jobs.empty()) {  // a job table (queue and far-queue offsets) is needed, in any order
+    wgm_batch_config cfg;
+    int grid = 0;
+    rc = normalize(rt, nullptr, &cfg);
+    if (!rc) rc = batch_grid(rt, cfg, &grid);
+    if (!rc) rc = build_jobs(b, cfg, grid);
+    if (rc) return rc;
+  }
+  const size_t hcap = (size_t)(*std::max_element(b->m.begin(), b->m.end()) / kHeavyMin) + 2;
+  if (b->sa_hcap < hcap) {
+    if (b->sa_heavy) (void)hipFree(b->sa_heavy);
+    b->sa_heavy = nullptr;
+    b->sa_hcap = 0;
+    HIPCHK(rt, hipMalloc(&b->sa_heavy, (3 * hcap + 3 * kLine) * sizeof(int32_t)));
+    b->sa_hcap = hcap;
+  }
+  const BatchDev d = dev_view(b);
+  hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, rt->stream, d);
+  HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+  for (int k = 0; k < b->count; k++) {
+    const BatchJob& j = b->jobs[k];
+    const bool sssp = j.query == WGM_QUERY_SSSP;
+    GraphDev g{};
+    g.row_ptr = b->row + j.row_off;
+    g.col = b->col + j.col_off;
+    g.wt = sssp ? b->wt + j.col_off : nullptr;
+    g.n = j.n;
+    g.m = j.m;
+    FrontierDev s{};
+    s.dist = b->dist + j.v_off;
+    s.q0 = b->q + j.q_off;
+    s.q1 = s.q0 + j.qcap;
+    s.qa0 = sssp ? b->qa + j.q_off : nullptr;
+    s.qa1 = sssp ? s.qa0 + j.qcap : nullptr;
+    s.counts = b->counts + (size_t)k * 3 * kLine;
+    s.claims = b->claims + (size_t)k * 3 * kLine;
+    s.hv = b->sa_heavy;
+    s.hdu = (uint32_t*)(b->sa_heavy + b->sa_hcap);
+    s.hclaim = b->sa_heavy + 2 * b->sa_hcap;
+    s.hcnt = b->sa_heavy + 3 * b->sa_hcap;
+    NearFar nf{};
+    if (sssp) {
+      nf.q[0] = b->far + j.f_off;
+      nf.q[1] = nf.q[0] + j.n + 1;
+      nf.flag = b->in_far + j.v_off;
+      nf.cnt = b->fcnt + (size_t)k * 3 * kLine;
+      nf.delta = wgm_standalone_delta(j.maxdeg);
+    }
+    rc = wgm_enqueue_reset_ctl(rt, rt->stream);
+    if (!rc) rc = wgm_enqueue_frontier(rt, g, s, nf, sssp, j.src, sssp ? grid_sssp : grid_bfs, wg, d.iters + k);
+    if (rc) return rc;
+  }
+  rc = enqueue_collect(b, rt->stream);
+  if (rc) return rc;
+  wgm_batch_stats local{};
+  local.launches = b->count;
+  local.grid = grid_bfs;
   rc = collect_results(b, &local);
   if (rc) return rc;
   local.device_ms = 0.0;  // per-launch clocks: only the wall time spans the whole baseline

@@ -115,6 +115,28 @@ int wgm_launch_standalone(wgm_runtime* rt, const void* kernel, int32_t wg_size, 
   return WGM_OK;
 }
 
+int wgm_discover_occupancy_custom(wgm_runtime* rt, const void* kernel, int32_t wg_size, int32_t lds_bytes,
+                                  const wgm_args* args, int32_t mega, int32_t* count) {
+  if (!rt || !kernel || !count) return fail(rt, WGM_ERR_INVALID, "NULL argument");
+  wgm_kernel_args wa{};
+  wgm_mega_args ma{};
+  std::vector<void*> params;
+  params.push_back(mega ? (void*)&ma : (void*)&wa);
+  if (args)
+    for (const auto& v : args->values) params.push_back((void*)v.data());
+  wa.ctl = rt->ctl;
+  wa.ctl.im_exit = 1;
+  ma.ctl = wa.ctl;  // im_exit: every workgroup returns right after discovery
+  return custom_grid(rt, kernel, wg_size, lds_bytes, 0, params.data(), count);
+}
+
+int wgm_mega_set_query_barrier(wgm_mega* m, int32_t query_barrier) {
+  if (!m) return fail(nullptr, WGM_ERR_INVALID, "NULL mega");
+  if (m->running) return fail(m->rt, WGM_ERR_STATE, "set the barrier form before the launch");
+  m->custom_query_barrier = query_barrier ? 1 : 0;
+  return WGM_OK;
+}
+
 int wgm_mega_launch_custom(wgm_mega* m, const void* kernel, int32_t wg_size, int32_t num_wgs, int32_t lds_bytes,
                            const wgm_args* args, int32_t* participating) {
   if (!m || !kernel) return fail(m ? m->rt : nullptr, WGM_ERR_INVALID, "NULL argument");
@@ -133,7 +155,7 @@ int wgm_mega_launch_custom(wgm_mega* m, const void* kernel, int32_t wg_size, int
   if (rc) return rc;
   if (grid < 2) return fail(rt, WGM_ERR_INVALID, "a mega-kernel needs at least 2 workgroups");
   m->wg_size = wg_size;
-  m->sd.query_barrier = 0;
+  m->sd.query_barrier = m->custom_query_barrier;
   m->sd.serve = 0;
   hstore(m->h_flag, DEVICE_SCHEDULER_INIT);
   hstore(m->h_persistent, -1);

@@ -9,6 +9,8 @@
 
 #include "runtime_internal.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -70,6 +72,30 @@ __global__ void __launch_bounds__(1024) k_barrier_bench(Ctl c, int iterations, i
     }
   }
   clock_end(c, id);
+}
+
+// HIP's own grid barrier as the reference point: a cooperative launch (every
+// workgroup co-resident by construction, no discovery) and
+// cooperative_groups' grid sync, 10 x iterations of them, with the same epoch
+// check and device-clock stamps as k_barrier_bench.
+__global__ void __launch_bounds__(1024) k_coop_barrier(int iterations, int check, int* errors, int* arrivals,
+                                                      unsigned long long* clock) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int n = (int)gridDim.x;
+  grid.sync();
+  if (blockIdx.x == 0 && threadIdx.x == 0) clock[0] = wall_clock64();
+  int b = 0;
+  for (int i = 0; i < iterations; i++) {
+    for (int j = 0; j < 10; j++, b++) {
+      if (check && threadIdx.x == 0) add_relaxed(arrivals, 1);
+      grid.sync();
+      if (check && threadIdx.x == 0) {
+        const int a = ld_relaxed(arrivals);
+        if (a < (b + 1) * n || a > (b + 2) * n) add_relaxed(errors, 1);
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) clock[1] = wall_clock64();
 }
 
 // Plain streaming copy (16 bytes per lane per step): the calibration kernel
@@ -432,7 +458,8 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size, int
 int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t num_wgs, int32_t iterations,
                       wgm_barrier_result* out) {
   if (!rt || !out) return fail(rt, WGM_ERR_INVALID, "NULL argument");
-  if (variant != WGM_BARRIER_FLAGS && variant != WGM_BARRIER_SENSE && variant != WGM_BARRIER_XCD)
+  if (variant != WGM_BARRIER_FLAGS && variant != WGM_BARRIER_SENSE && variant != WGM_BARRIER_XCD &&
+      variant != WGM_BARRIER_COOP)
     return fail(rt, WGM_ERR_INVALID, "unknown barrier variant %d", variant);
   if (iterations < 0) return fail(rt, WGM_ERR_INVALID, "negative iterations");
   int grid = 0;
@@ -440,6 +467,52 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
   if (rc) return rc;
   int* d_aux = nullptr;
   HIPCHK(rt, hipMalloc(&d_aux, 2 * sizeof(int)));
+  if (variant == WGM_BARRIER_COOP) {
+    // The grid is never above what the discovery probe found co-resident for
+    // k_barrier_bench (which uses more registers and LDS than this kernel),
+    // and HIP rejects a cooperative grid it cannot make co-resident.
+    int per_cu = 0;
+    rc = wgm_occupancy(rt, WGM_K_BARRIER_BENCH, wg_size, &per_cu);
+    if (rc == WGM_OK && num_wgs > 0) {
+      int found = 0;
+      rc = wgm_resolve_grid(rt, WGM_K_BARRIER_BENCH, wg_size, 0, &found);
+      if (rc == WGM_OK && grid > found) rc = fail(rt, WGM_ERR_TOO_MANY_GROUPS, "cooperative grid %d above the %d found co-resident", grid, found);
+    }
+    auto run_coop = [&](int check, int iters, wgm_barrier_result* r) -> int {
+      HIPCHK(rt, hipMemsetAsync(d_aux, 0, 2 * sizeof(int), rt->stream));
+      HIPCHK(rt, hipMemsetAsync(rt->ctl.clock, 0, 2 * sizeof(unsigned long long), rt->stream));
+      int* errs_p = d_aux;
+      int* arr_p = d_aux + 1;
+      unsigned long long* clk = rt->ctl.clock;
+      void* params[] = {&iters, &check, &errs_p, &arr_p, &clk};
+      HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
+      HIPCHK(rt, hipLaunchCooperativeKernel((const void*)&k_coop_barrier, dim3(grid), dim3(wg_size), params, 0,
+                                            rt->stream));
+      HIPCHK(rt, hipEventRecord(rt->ev1, rt->stream));
+      HIPCHK(rt, hipEventSynchronize(rt->ev1));
+      float ms = 0.f;
+      HIPCHK(rt, hipEventElapsedTime(&ms, rt->ev0, rt->ev1));
+      unsigned long long c[2] = {0, 0};
+      HIPCHK(rt, hipMemcpy(c, rt->ctl.clock, sizeof(c), hipMemcpyDeviceToHost));
+      int errs = 0;
+      HIPCHK(rt, hipMemcpy(&errs, d_aux, sizeof(int), hipMemcpyDeviceToHost));
+      r->participating = grid;
+      r->barriers = iters * 10;
+      r->epoch_errors = errs;
+      r->kernel_ms = ms;
+      r->device_ms = c[1] >= c[0] ? (double)(c[1] - c[0]) * 1e3 / (double)rt->info.wall_clock_hz : 0.0;
+      r->ns_per_barrier = r->barriers ? r->device_ms * 1e6 / r->barriers : 0.0;
+      return WGM_OK;
+    };
+    wgm_barrier_result checked{}, timed{};
+    if (rc == WGM_OK) rc = run_coop(1, iterations < 10 ? iterations : 10, &checked);
+    if (rc == WGM_OK) rc = run_coop(0, iterations, &timed);
+    (void)hipFree(d_aux);
+    if (rc) return rc;
+    timed.epoch_errors = checked.epoch_errors;
+    *out = timed;
+    return WGM_OK;
+  }
   auto run = [&](int check, int iters, wgm_barrier_result* r) -> int {
     HIPCHK(rt, hipMemsetAsync(d_aux, 0, 2 * sizeof(int), rt->stream));
     int rc2 = wgm_reset_ctl(rt, 0, 0);

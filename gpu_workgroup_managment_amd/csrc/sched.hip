@@ -65,6 +65,150 @@ __device__ void p_frontier(const SchedDev& s, const MegaApp& a, RCtx rin, int di
   if (vid == 0 && threadIdx.x == 0) *a.iters = level;
 }
 
+// ------------------------------------------ resizable topology-driven tasks
+// Colour, MIS and PageRank under the resizing scheduler (merged color / mis,
+// src/pannotia/graph_apps/color/device/merged.cl:73-205,
+// src/pannotia/graph_apps/mis/device/merged.cl:75-224). Every phase ends in a
+// resizing barrier, the restoration context is {resume label, round}, and
+// the vertices of a phase are claimed in chunks from a counter, so a
+// workgroup forked in at a barrier or killed at one changes nothing in how
+// the phase is split. The per-vertex results do not depend on the split: a
+// round's outcome is fixed by the states at its start (colour, MIS), and a
+// PageRank sum is taken by one thread in CSR order. Claim counters rotate
+// over three lines by phase index (phase p uses line p % 3, and workgroup 0
+// clears line (p + 1) % 3, whose last users finished two barriers ago); the
+// stop flags rotate the same way by round.
+constexpr int kPhaseChunk = 2048;  // vertices per claim
+
+template <class Body>
+__device__ __forceinline__ void claim_chunks(int n, int* claim, Body&& body) {
+  __shared__ int s_c;
+  while (true) {
+    if (threadIdx.x == 0) s_c = add_relaxed(claim, 1);
+    __syncthreads();
+    const long long lo = (long long)__builtin_amdgcn_readfirstlane(s_c) * kPhaseChunk;
+    if (lo >= n) break;
+    body((int)lo, (int)(lo + kPhaseChunk < n ? lo + kPhaseChunk : n));
+    __syncthreads();  // the body ends in a barrier: the next claim by thread 0 stays one loop
+  }
+  __syncthreads();
+}
+
+// reset_persistent_task on the device: static blocks of the initial team
+// (no resize happens before the first barrier), and workgroup 0 clears the
+// claim and stop lines.
+template <class Init>
+__device__ __forceinline__ void topo_init(const SchedDev& s, const MegaApp& a, int vid, Init&& init) {
+  const int k0 = ld_relaxed(s.bar_groups);
+  int lo, hi;
+  block_range(a.g.n, vid, k0, &lo, &hi);
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) init(v);
+  if (vid == 0 && threadIdx.x == 0) {
+    for (int l = 0; l < 3; l++) {
+      st_relaxed(&a.claims[l * kLine], 0);
+      st_relaxed(&a.flags[l * kLine], 0);
+    }
+  }
+}
+
+// Colour: one resize point per round (the fused round of color_round; the
+// reference has two, after color1 and after color2). Resume label 1 = right
+// after the barrier that ends round `level` (0: the initialisation).
+__device__ void p_color(const SchedDev& s, const MegaApp& a, RCtx rin, int did) {
+  const int vid = s.p.group_ids[did];
+  int round = 0;
+  if (rin.target == 0) {
+    topo_init(s, a, vid, [&](int v) { a.ia[v] = -1; });
+    if (resize_barrier(s, vid, RCtx{1, 0})) return;
+  } else {
+    round = rin.level;
+  }
+  while (true) {
+    if (round > 0 && ld_relaxed(&a.flags[(round % 3) * kLine]) == 0) break;
+    round++;
+    bool any = false;
+    claim_chunks(a.g.n, &a.claims[(round % 3) * kLine],
+                 [&](int lo, int hi) { any = color_round(a.g, a.ia, round, lo, hi) || any; });
+    if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&a.flags[(round % 3) * kLine], 1);
+    if (vid == 0 && threadIdx.x == 0) {
+      st_relaxed(&a.flags[((round + 1) % 3) * kLine], 0);
+      st_relaxed(&a.claims[((round + 1) % 3) * kLine], 0);
+    }
+    if (resize_barrier(s, vid, RCtx{1, round})) return;
+  }
+  if (vid == 0 && threadIdx.x == 0) *a.iters = round;
+}
+
+// MIS: two resize points per round (phase A = mis1 + the join of mis2,
+// phase B = the removals of mis2 + mis3; the reference has three). Resume
+// label 1 = after phase A of round `level`, 2 = after phase B (0: after the
+// initialisation). Phase A of round r is phase 2r - 1, phase B is 2r.
+__device__ void p_mis(const SchedDev& s, const MegaApp& a, RCtx rin, int did) {
+  const int vid = s.p.group_ids[did];
+  int round = 0, at = 2;
+  if (rin.target == 0) {
+    topo_init(s, a, vid, [&](int v) {
+      a.ia[v] = -1;
+      a.ib[v] = 0;
+    });
+    if (resize_barrier(s, vid, RCtx{2, 0})) return;
+  } else {
+    round = rin.level;
+    at = rin.target;
+  }
+  while (true) {
+    if (at == 2) {
+      round++;
+      bool any = false;
+      claim_chunks(a.g.n, &a.claims[((2 * round - 1) % 3) * kLine],
+                   [&](int lo, int hi) { any = mis_phase_a(a.g, a.ia, a.ib, round, lo, hi) || any; });
+      if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&a.flags[(round % 3) * kLine], 1);
+      if (vid == 0 && threadIdx.x == 0) {
+        st_relaxed(&a.flags[((round + 1) % 3) * kLine], 0);
+        st_relaxed(&a.claims[((2 * round) % 3) * kLine], 0);
+      }
+      if (resize_barrier(s, vid, RCtx{1, round})) return;
+      at = 1;
+    } else {
+      if (ld_relaxed(&a.flags[(round % 3) * kLine]) == 0) break;
+      claim_chunks(a.g.n, &a.claims[((2 * round) % 3) * kLine],
+                   [&](int lo, int hi) { mis_phase_b(a.g, a.ia, round, lo, hi); });
+      if (vid == 0 && threadIdx.x == 0) st_relaxed(&a.claims[((2 * round + 1) % 3) * kLine], 0);
+      if (resize_barrier(s, vid, RCtx{2, round})) return;
+      at = 2;
+    }
+  }
+  if (vid == 0 && threadIdx.x == 0) *a.iters = round;
+}
+
+// PageRank: one resize point per iteration; resume label 1 with the number
+// of iterations done (0: after the initial contributions). Iteration i is
+// phase i + 1.
+__device__ void p_pagerank(const SchedDev& s, const MegaApp& a, RCtx rin, int did) {
+  const int vid = s.p.group_ids[did];
+  int it = 0;
+  if (rin.target == 0) {
+    topo_init(s, a, vid, [&](int v) {
+      const int deg = a.g.row_ptr[v + 1] - a.g.row_ptr[v];
+      a.fa[v] = deg ? a.r0 / (float)deg : 0.0f;
+    });
+    if (resize_barrier(s, vid, RCtx{1, 0})) return;
+  } else {
+    it = rin.level;
+  }
+  while (it < a.iterations) {
+    const float* cin = (it & 1) ? a.fb : a.fa;
+    float* cout = (it & 1) ? a.fa : a.fb;
+    const bool last = it == a.iterations - 1;
+    claim_chunks(a.g.n, &a.claims[((it + 1) % 3) * kLine],
+                 [&](int lo, int hi) { pagerank_iter(a.g, cin, cout, a.fc, a.damping, a.base, last, lo, hi); });
+    if (vid == 0 && threadIdx.x == 0) st_relaxed(&a.claims[((it + 2) % 3) * kLine], 0);
+    it++;
+    if (resize_barrier(s, vid, RCtx{1, it})) return;
+  }
+  if (vid == 0 && threadIdx.x == 0) *a.iters = it;
+}
+
 // ======================================================= the mega-kernel
 // src/scheduler_rt/rt_device/main_device_body.cl:1-93
 __global__ void __launch_bounds__(1024) k_mega(Ctl c, SchedDev s, const MegaApp* app_cell, NPDev np) {
@@ -93,8 +237,13 @@ __global__ void __launch_bounds__(1024) k_mega(Ctl c, SchedDev s, const MegaApp*
     } else if (task == TASK_PERSIST) {
       const MegaApp app = read_app(app_cell);
       const RCtx r = *rslot<RCtx>(s.r_ctx, did);
-      if (app.kind == WGM_PERSIST_SSSP) p_frontier<true>(s, app, r, did, lds);
-      else p_frontier<false>(s, app, r, did, lds);
+      switch (app.kind) {
+        case WGM_PERSIST_SSSP: p_frontier<true>(s, app, r, did, lds); break;
+        case WGM_PERSIST_COLOR: p_color(s, app, r, did); break;
+        case WGM_PERSIST_MIS: p_mis(s, app, r, did); break;
+        case WGM_PERSIST_PAGERANK: p_pagerank(s, app, r, did); break;
+        default: p_frontier<false>(s, app, r, did, lds); break;
+      }
       drain_vmem();
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -466,35 +615,63 @@ int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query
   return WGM_OK;
 }
 
-// Persistent BFS / SSSP task. Its descriptor goes to coherent host memory;
-// the initial team resets the traversal state on the device.
-int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups) {
-  if (!m || !g) return fail(m ? m->rt : nullptr, WGM_ERR_INVALID, "NULL argument");
+// A persistent task: BFS / SSSP (frontier), colour, MIS or PageRank. Its
+// descriptor goes to coherent host memory; the initial team resets the
+// task's state on the device.
+int wgm_mega_send_persistent_task(wgm_mega* m, wgm_graph* g, const wgm_persistent_task* t, int32_t groups) {
+  if (!m || !g || !t) return fail(m ? m->rt : nullptr, WGM_ERR_INVALID, "NULL argument");
   wgm_runtime* rt = m->rt;
   if (!m->running) return fail(rt, WGM_ERR_STATE, "mega-kernel not running");
   if (groups < 1 || groups > m->participating)
     return fail(rt, WGM_ERR_TOO_MANY_GROUPS, "cannot send a persistent task with %d workgroups (%d participate)",
                 groups, m->participating);
-  if (kind != WGM_PERSIST_BFS && kind != WGM_PERSIST_SSSP) return fail(rt, WGM_ERR_INVALID, "unknown persistent kind");
+  const int kind = t->kind;
+  if (kind < WGM_PERSIST_BFS || kind > WGM_PERSIST_PAGERANK) return fail(rt, WGM_ERR_INVALID, "unknown persistent kind");
   if (kind == WGM_PERSIST_SSSP && !g->wt) return fail(rt, WGM_ERR_INVALID, "SSSP needs weights");
-  if (src < 0 || src >= g->n) return fail(rt, WGM_ERR_INVALID, "bad source");
+  if ((kind == WGM_PERSIST_COLOR || kind == WGM_PERSIST_MIS) && !g->prio)
+    return fail(rt, WGM_ERR_INVALID, "colouring and MIS need vertex priorities");
+  if (kind <= WGM_PERSIST_SSSP && (t->src < 0 || t->src >= g->n)) return fail(rt, WGM_ERR_INVALID, "bad source");
+  if (kind == WGM_PERSIST_PAGERANK && (t->iterations < 1 || !(t->damping >= 0.0f && t->damping <= 1.0f)))
+    return fail(rt, WGM_ERR_INVALID, "PageRank needs iterations >= 1 and damping in [0, 1]");
   if (m->monitor.joinable()) m->monitor.join();
   FrontierDev f{};
   int rc = wgm_graph_frontier_state(g, kind == WGM_PERSIST_SSSP, &f);
+  if (!rc) rc = wgm_graph_app_state(g, kind == WGM_PERSIST_COLOR ? 1 : kind == WGM_PERSIST_MIS ? 2 :
+                                          kind == WGM_PERSIST_PAGERANK ? 3 : 0);
   if (rc) return rc;
-  if (!m->claims) HIPCHK(rt, hipMalloc(&m->claims, 3 * kLine * sizeof(int)));
+  if (!m->claims) {
+    HIPCHK(rt, hipMalloc(&m->claims, 3 * kLine * sizeof(int)));
+    HIPCHK(rt, hipMemset(m->claims, 0, 3 * kLine * sizeof(int)));
+  }
   f.claims = (int*)m->claims;
   f.chunk = m->wg_size;
   MegaApp app{};
   app.kind = kind;
-  app.src = src;
+  app.src = t->src;
   app.g = wgm_graph_view(g);
   app.f = f;
   app.iters = g->iters;
+  app.ia = g->i32a;
+  app.ib = g->i32b;
+  app.fa = g->f0;
+  app.fb = g->f1;
+  app.fc = g->f2;
+  app.flags = f.counts;
+  app.claims = (int*)m->claims;
+  app.damping = t->damping;
+  app.base = (1.0f - t->damping) / (float)g->n;
+  app.r0 = 1.0f / (float)g->n;
+  app.iterations = t->iterations;
   std::memcpy((void*)m->h_app, &app, sizeof(app));
   std::atomic_thread_fence(std::memory_order_release);
   m->graph = g;
+  m->app_kind = kind;
   return mega_start_persistent(m, groups);
+}
+
+int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups) {
+  const wgm_persistent_task t{kind, src, 0.85f, 50};
+  return wgm_mega_send_persistent_task(m, g, &t, groups);
 }
 
 }  // extern "C"
@@ -614,11 +791,15 @@ int wgm_mega_get_stats(wgm_mega* m, wgm_mega_stats* st) {
 
 // check_persistent_task: the result is read after the mega-kernel has quit
 // (a copy may need the GPU, which the resident mega-kernel holds).
-int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels) {
+int wgm_mega_result(wgm_mega* m, void* out, int32_t* levels) {
   if (!m || !m->graph) return fail(m ? m->rt : nullptr, WGM_ERR_STATE, "no persistent task has run");
   wgm_runtime* rt = m->rt;
   if (m->running) return fail(rt, WGM_ERR_STATE, "read results after wgm_mega_quit");
-  if (dist_out) HIPCHK(rt, hipMemcpy(dist_out, m->graph->dist, sizeof(uint32_t) * m->graph->n, hipMemcpyDeviceToHost));
+  const wgm_graph* g = m->graph;
+  const void* src = m->app_kind == WGM_PERSIST_COLOR ? (const void*)g->i32a
+                    : m->app_kind == WGM_PERSIST_MIS ? (const void*)g->i32b
+                    : m->app_kind == WGM_PERSIST_PAGERANK ? (const void*)g->f2 : (const void*)g->dist;
+  if (out) HIPCHK(rt, hipMemcpy(out, src, 4 * (size_t)g->n, hipMemcpyDeviceToHost));
   if (levels) HIPCHK(rt, hipMemcpy(levels, m->graph->iters, sizeof(int), hipMemcpyDeviceToHost));
   return WGM_OK;
 }

@@ -23,6 +23,17 @@ struct MegaApp {
   GraphDev g;
   FrontierDev f;
   int* iters;
+  // topology-driven tasks (colour, MIS, PageRank): outputs and per-vertex
+  // state, stop-flag lines (3, by round % 3) and claim lines (3, by phase % 3)
+  int32_t* ia;   // colour / MIS state
+  int32_t* ib;   // MIS membership (2 = in the set, 0 = not)
+  float* fa;     // PageRank contributions, even iterations
+  float* fb;     // PageRank contributions, odd iterations
+  float* fc;     // PageRank result
+  int* flags;
+  int* claims;
+  float damping, base, r0;
+  int iterations;
 };
 
 // Copy the descriptor out of host memory with system-scope loads.
@@ -112,6 +123,8 @@ struct wgm_mega {
   wgm::MegaApp* h_app = nullptr;
   wgm::MegaApp* h_app_dev = nullptr;
   wgm_graph* graph = nullptr;
+  int app_kind = 0;              // WGM_PERSIST_* of the last persistent task
+  int custom_query_barrier = 0;  // barrier form of a caller's mega-kernel
   int wg_size = 256;
   bool running = false;
   int participating = 0;

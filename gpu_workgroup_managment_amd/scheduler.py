@@ -9,7 +9,9 @@ import numpy as np
 from . import _lib as L
 
 NP_MATMULT_I32, NP_MATMUL_F32, NP_REDUCE_MIN = 0, 1, 2
-PERSIST_BFS, PERSIST_SSSP = 0, 1
+PERSIST_BFS, PERSIST_SSSP, PERSIST_COLOR, PERSIST_MIS, PERSIST_PAGERANK = 0, 1, 2, 3, 4
+_RESULT_DTYPE = {PERSIST_BFS: np.uint32, PERSIST_SSSP: np.uint32, PERSIST_COLOR: np.int32, PERSIST_MIS: np.int32,
+                 PERSIST_PAGERANK: np.float32}
 
 
 def np_config(kind=NP_MATMUL_F32, dim=256, length=1 << 20, seed=0):
@@ -55,9 +57,12 @@ class Mega:
         self.participating = p.value
         return p.value
 
-    def send_persistent(self, graph, kind=PERSIST_BFS, src=0, groups=None):
+    def send_persistent(self, graph, kind=PERSIST_BFS, src=0, groups=None, damping=0.85, iterations=50):
         groups = self.participating if groups is None else groups
-        self.rt._check(L.lib().wgm_mega_send_persistent(self._h, graph._h, kind, src, groups))
+        t = L.PersistentTask()
+        t.kind, t.src, t.damping, t.iterations = kind, src, damping, iterations
+        self._kind = kind
+        self.rt._check(L.lib().wgm_mega_send_persistent_task(self._h, graph._h, ctypes.byref(t), groups))
 
     def persistent_running(self):
         return bool(L.lib().wgm_mega_persistent_running(self._h))
@@ -83,10 +88,12 @@ class Mega:
         return s.as_dict()
 
     def result(self, n):
-        dist = np.empty(n, np.uint32)
+        """(output, levels / rounds / iterations) of the last persistent task:
+        distances, colours, MIS membership or ranks."""
+        out = np.empty(n, _RESULT_DTYPE[getattr(self, "_kind", PERSIST_BFS)])
         lv = ctypes.c_int32()
-        self.rt._check(L.lib().wgm_mega_result(self._h, L.ptr(dist), ctypes.byref(lv)))
-        return dist, lv.value
+        self.rt._check(L.lib().wgm_mega_result(self._h, L.ptr(out), ctypes.byref(lv)))
+        return out, lv.value
 
     def launch_custom(self, kernel, args=None, wg_size=256, num_wgs=0, lds_bytes=0):
         """Launch the caller's merged mega-kernel (first parameter wgm_mega_args)."""

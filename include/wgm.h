@@ -196,6 +196,8 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
 #define WGM_BARRIER_FLAGS 0   /* flat master/slave flag barrier, one line per WG */
 #define WGM_BARRIER_SENSE 1   /* counter + sense reversal */
 #define WGM_BARRIER_XCD 2     /* two-level: members -> XCD leader, leaders <-> leaders */
+#define WGM_BARRIER_COOP 3    /* reference point: hipLaunchCooperativeKernel + cooperative_groups grid sync
+                                 (the grid is at most the discovered occupancy) */
 typedef struct wgm_barrier_result {
   int32_t participating;
   int32_t barriers;
@@ -296,6 +298,17 @@ typedef struct wgm_mega wgm_mega;
 
 #define WGM_PERSIST_BFS 0
 #define WGM_PERSIST_SSSP 1
+#define WGM_PERSIST_COLOR 2     /* Pannotia colour (merged color, color/device/merged.cl:73-205) */
+#define WGM_PERSIST_MIS 3       /* Pannotia MIS (merged mis, mis/device/merged.cl:75-224) */
+#define WGM_PERSIST_PAGERANK 4  /* pull PageRank, FP32 (no counterpart in the reference) */
+
+/* A persistent task for the resizing scheduler. */
+typedef struct wgm_persistent_task {
+  int32_t kind;        /* WGM_PERSIST_* */
+  int32_t src;         /* BFS / SSSP source vertex */
+  float damping;       /* PageRank */
+  int32_t iterations;  /* PageRank */
+} wgm_persistent_task;
 
 #define WGM_NP_MATMULT_I32 0  /* reference matmult: int32, djb2 hash of the diagonal */
 #define WGM_NP_MATMUL_F32 1   /* FP32 dim x dim (dim % 32 == 0), FP64 checksum check */
@@ -341,10 +354,16 @@ int wgm_mega_destroy(wgm_mega* m);
  * Replaces CL_Communicator::launch_mega_kernel / number_of_discovered_groups
  * (src/scheduler_rt/rt_host/cl_communicator.h:71-99). */
 int wgm_mega_launch(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t query_barrier, int32_t* participating);
-/* Start a persistent BFS / SSSP on `groups` workgroups (returns once
- * assigned; a monitor thread samples the executing workgroups every 1 ms).
- * Replaces send_persistent_task / monitor_persistent_task
- * (src/scheduler_rt/rt_host/cl_communicator.h:168-224). */
+/* Start a persistent task on `groups` workgroups (returns once assigned; a
+ * monitor thread samples the executing workgroups every 1 ms). Replaces
+ * send_persistent_task / monitor_persistent_task
+ * (src/scheduler_rt/rt_host/cl_communicator.h:168-224). Every task gives up
+ * and regains workgroups at its resizing barriers: BFS / SSSP one per level
+ * (merged drelax2, src/lonestar/graph_apps/bfs/device/merged.cl:310-396),
+ * colour one per round, MIS two per round, PageRank one per iteration.
+ * Colour and MIS need the graph's priorities. */
+int wgm_mega_send_persistent_task(wgm_mega* m, wgm_graph* g, const wgm_persistent_task* t, int32_t groups);
+/* The same with kind and source (PageRank: damping 0.85, 50 iterations). */
 int wgm_mega_send_persistent(wgm_mega* m, wgm_graph* g, int32_t kind, int32_t src, int32_t groups);
 /* is_executing_persistent (src/scheduler_rt/rt_host/cl_communicator.h:226-228). */
 int wgm_mega_persistent_running(wgm_mega* m);
@@ -363,8 +382,10 @@ int wgm_mega_quit(wgm_mega* m);
 /* The summary of print_summary (src/scheduler_rt/rt_host/cl_communicator.h:296-348),
  * plus p50 / p99. */
 int wgm_mega_get_stats(wgm_mega* m, wgm_mega_stats* out);
-/* Result of the last persistent task (after wgm_mega_quit). */
-int wgm_mega_result(wgm_mega* m, uint32_t* dist_out, int32_t* levels);
+/* Result of the last persistent task (after wgm_mega_quit), n 4-byte values:
+ * uint32 distances (BFS / SSSP), int32 colours, int32 MIS membership (2 in
+ * the set, 0 not) or FP32 ranks; *levels = levels / rounds / iterations. */
+int wgm_mega_result(wgm_mega* m, void* out, int32_t* levels);
 /* The non-persistent task launched on its own, `iterations` times:
  * run_non_persistent (src/host_skeleton/skel.cpp:168-221). */
 int wgm_np_run_solo(wgm_mega* m, int32_t wg_size, int32_t num_wgs, int32_t iterations, double* mean_us,
@@ -419,6 +440,109 @@ int wgm_mega_send_persistent_custom(wgm_mega* m, int32_t groups);
 /* send_task_synchronous for the kernel's own non-persistent task
  * (cl_communicator.h:123-166); the caller checks the task's output. */
 int wgm_mega_send_task_custom(wgm_mega* m, int32_t groups, wgm_task_times* out);
+
+/* Workgroups of `kernel` (the caller's standalone kernel, first parameter
+ * wgm_kernel_args; or with mega = 1 a merged mega-kernel, first parameter
+ * wgm_mega_args) that a discovery probe finds co-resident, launched with
+ * `args`: get_occupancy_d_ctx on the app's own kernel (skel.cpp:223-240). */
+int wgm_discover_occupancy_custom(wgm_runtime* rt, const void* kernel, int32_t wg_size, int32_t lds_bytes,
+                                  const wgm_args* args, int32_t mega, int32_t* count);
+/* Resizing barriers of the next wgm_mega_launch_custom in the query form
+ * (__global_barrier_resize_query, src/scheduler_rt/rt_device/iw_barrier.cl:229-305):
+ * the reference's --use_query_barrier. */
+int wgm_mega_set_query_barrier(wgm_mega* m, int32_t query_barrier);
+
+/* ---------------------------------------------------------- host skeleton
+ * The reference's driver picks one persistent app and one non-persistent
+ * task at compile time (src/host_skeleton/skel.cpp:47-77) and calls their
+ * free functions. Here the two plugin interfaces are vtables of the same
+ * entry points, each taking the plugin's own state; kernel arguments go
+ * through the ordered wgm_args builder instead of cl::Kernel::setArg with a
+ * running index, and every entry point returns a status. */
+typedef struct wgm_persistent_app {
+  void* self;
+  /* persistent_app_name (src/lonestar/graph_apps/bfs/bfs.h:192-194) */
+  const char* (*name)(void* self);
+  /* persistent_kernel_name (bfs.h:196-198): the host handle of the
+   * standalone kernel (first parameter wgm_kernel_args) */
+  const void* (*kernel)(void* self);
+  /* init_persistent_app_for_occupancy (bfs.h:209-246): read and upload */
+  int (*init_for_occupancy)(void* self, wgm_runtime* rt);
+  /* init_persistent_app_for_real (bfs.h:248-263): size per-run state for
+   * `occupancy` workgroups */
+  int (*init_for_real)(void* self, wgm_runtime* rt, int32_t occupancy);
+  /* set_persistent_app_args_for_occupancy / _for_real (bfs.h:265-283) */
+  int (*set_args_for_occupancy)(void* self, wgm_args* args);
+  int (*set_args_for_real)(void* self, wgm_args* args);
+  /* reset_persistent_task (bfs.h:284-313) */
+  int (*reset)(void* self, wgm_runtime* rt);
+  /* check_persistent_task (bfs.h:315-334): WGM_OK, or WGM_ERR_MISMATCH */
+  int (*check)(void* self, wgm_runtime* rt);
+  /* clean_persistent_task (bfs.h:366-370) */
+  int (*clean)(void* self, wgm_runtime* rt);
+} wgm_persistent_app;
+
+typedef struct wgm_np_task {
+  void* self;
+  /* non_persistent_app_name (src/non_persistent_kernels/matmult/matmult.h:26-28) */
+  const char* (*name)(void* self);
+  /* non_persistent_kernel_name (matmult.h:30-32): the kernel's standalone
+   * handle, or NULL (the merged mode runs the task inside the mega-kernel) */
+  const void* (*kernel)(void* self);
+  /* init_non_persistent_app (matmult.h:94-164) */
+  int (*init)(void* self, wgm_runtime* rt);
+  /* set_non_persistent_app_args (matmult.h:166-184) */
+  int (*set_args)(void* self, wgm_args* args);
+  /* reset_non_persistent (matmult.h:186-188); runs while the mega-kernel
+   * holds the GPU, so it may touch host memory only */
+  int (*reset)(void* self);
+  /* check_non_persistent_task (matmult.h:190-194): WGM_OK or WGM_ERR_MISMATCH */
+  int (*check)(void* self);
+  /* clean_non_persistent_task (matmult.h:196-200) */
+  int (*clean)(void* self, wgm_runtime* rt);
+} wgm_np_task;
+
+/* The reference's flags for these run modes (skel.cpp:21-44). */
+typedef struct wgm_skel_config {
+  int32_t threads_per_wg;       /* FLAGS_threads_per_wg (default 128) */
+  int32_t num_wgs;              /* FLAGS_num_wgs: at most this many (<= 0: the discovered occupancy) */
+  int32_t iterations;           /* FLAGS_run_persistent / FLAGS_merged_iterations (default 1) */
+  int32_t non_persistent_wgs;   /* FLAGS_non_persistent_wgs: -1 occupancy - 2, -2 one, k occupancy / k */
+  double non_persistent_frequency_ms;  /* FLAGS_non_persistent_frequency: task period */
+  int32_t skip_tasks;           /* FLAGS_skip_tasks */
+  int32_t use_query_barrier;    /* FLAGS_use_query_barrier */
+  int32_t lds_bytes;            /* dynamic LDS of the kernel */
+  int32_t max_tasks;            /* cap on tasks per merged iteration (<= 0: none) */
+} wgm_skel_config;
+
+typedef struct wgm_skel_report {
+  int32_t occupancy;            /* discovered co-resident workgroups */
+  int32_t num_wgs;              /* final size: min(num_wgs, occupancy) */
+  int32_t np_wgs;               /* workgroups per non-persistent task */
+  int32_t iterations;
+  int32_t persistent_errors;    /* iterations whose check_persistent_task failed */
+  int32_t np_errors;            /* tasks whose check_non_persistent_task failed */
+  int32_t tasks;                /* non-persistent tasks sent */
+  int32_t redlines;             /* tasks sent late (the period had already passed) */
+  double total_ms, mean_ms, min_ms, max_ms, std_ms;  /* persistent task time per iteration */
+  double response_p50_us, response_p99_us, exec_p50_us, exec_p99_us;  /* non-persistent tasks (merged) */
+} wgm_skel_report;
+
+/* run_persistent (skel.cpp:250-385): init for occupancy, set args, discover
+ * the occupancy of the app's kernel, init for real on min(num_wgs,
+ * occupancy), set args, then per iteration reset, launch with the host
+ * handshake (time = flag flip -> every workgroup retired), check; clean. */
+int wgm_run_persistent(wgm_runtime* rt, const wgm_persistent_app* app, const wgm_skel_config* cfg,
+                       wgm_skel_report* out);
+/* run_merged + execute_merged_iteration (skel.cpp:405-631): the same set-up
+ * with the non-persistent task's arguments first, on `merged_kernel` (the
+ * kernel_merge output of the two kernels, first parameter wgm_mega_args);
+ * per iteration reset both, launch, send the persistent task to every
+ * participant, then one non-persistent task per period (redlines counted)
+ * with a reset before and a check after each, until the persistent task
+ * ends; quit and check. */
+int wgm_run_merged(wgm_runtime* rt, const wgm_persistent_app* app, const wgm_np_task* np, const void* merged_kernel,
+                   const wgm_skel_config* cfg, wgm_skel_report* out);
 
 /* --------------------------------------------------------- query server
  * Serving (no counterpart in the reference, whose driver sends one
@@ -543,6 +667,8 @@ typedef struct wgm_batch_config {
   int32_t delta_high_degree; /* SSSP band width on graphs with an out-degree above 4 (0: delta):
                                 their few wide levels favour narrow bands (less rework), the
                                 long narrow traversals of low-degree graphs wide ones (fewer levels) */
+  int32_t far_flags_lds;     /* 1: a low-degree SSSP query on one 1024-thread workgroup keeps its near-far
+                                flags in an LDS bitmap (graphs up to 2^18 vertices) instead of HBM */
 } wgm_batch_config;
 
 typedef struct wgm_batch_stats {
@@ -568,6 +694,10 @@ int64_t wgm_batch_bytes(const wgm_batch* b);
 int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
 /* Baseline: one persistent launch per graph, back to back. */
 int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
+/* Baseline: one launch per query of the standalone app kernels of wgm_bfs /
+ * wgm_sssp (every discovered workgroup as one team; lc->num_wgs <= 0: the
+ * discovered occupancy), back to back, on the batch's resident graphs. */
+int wgm_batch_run_standalone(wgm_runtime* rt, wgm_batch* b, const wgm_launch* lc, wgm_batch_stats* st);
 /* Result of query `index` of the last run (any output may be NULL). */
 int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* iterations, int64_t* edges_scanned);
 /* Per-query start and end of the last run (wgm_batch_run or _relaunch), in

@@ -42,6 +42,7 @@ def lib():
             "or_hash_diag": (i32, [_i32p, i32, i32]),
             "or_matmul_f64": (None, [_f32p, _f32p, _f64p, i32, i32, i32]),
             "or_reduce_min": (i32, [_i32p, i64]),
+            "or_threads_pin": (i32, [i32]),
             "or_bfs_threads": (i32, [i32, _i32p, _i32p, i32, _u32p, i32]),
             "or_sssp_threads": (i32, [i32, _i32p, _i32p, _i32p, i32, _u32p, i32]),
             "or_fnv1a64": (ctypes.c_uint64, [ctypes.c_void_p, i64]),
@@ -143,6 +144,11 @@ def matmul_f64(a, b):
 def reduce_min(buf):
     buf = np.ascontiguousarray(buf, np.int32)
     return lib().or_reduce_min(buf, len(buf))
+
+
+def threads_pin(on=True):
+    """Pin the threaded restatement's threads, one per CPU of the caller's set."""
+    lib().or_threads_pin(1 if on else 0)
 
 
 def bfs_threads(row_ptr, col, src=0, threads=8):

@@ -6,10 +6,12 @@
  *
  * Plain C99 + pthreads. Build: oracle/Makefile -> oracle/_build/liboracle.so.
  */
+#define _GNU_SOURCE  /* pthread_setaffinity_np / CPU_SET for the pinned threaded restatement */
 #include "wgm_oracle.h"
 
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -365,11 +367,28 @@ typedef struct {
   pthread_barrier_t bar;
 } th_shared;
 
-typedef struct { th_shared* sh; int32_t tid; } th_arg;
+typedef struct { th_shared* sh; int32_t tid; int32_t cpu; } th_arg;
+
+/* Pinning of the threaded restatement: 0 leaves threads to the scheduler;
+ * 1 pins thread t to the t-th CPU of the caller's affinity set. */
+static int g_pin = 0;
+int or_threads_pin(int32_t on) {
+  g_pin = on ? 1 : 0;
+  return 0;
+}
+
+static void pin_to(int32_t cpu) {
+  if (cpu < 0) return;
+  cpu_set_t set;
+  CPU_ZERO(&set);
+  CPU_SET(cpu, &set);
+  pthread_setaffinity_np(pthread_self(), sizeof(set), &set);
+}
 
 static void* th_frontier(void* p) {
   th_arg* a = (th_arg*)p;
   th_shared* s = a->sh;
+  pin_to(a->cpu);
   const int sssp = s->wt != NULL;
   int32_t* local = (int32_t*)malloc(sizeof(int32_t) * 4096);
   int cap = 4096;
@@ -454,13 +473,30 @@ static int run_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, co
   s.q[0][0] = src;
   s.count[0] = 1;
   pthread_barrier_init(&s.bar, NULL, (unsigned)threads);
-  for (int32_t t = 0; t < threads; t++) {
+  /* pinned: thread t on the t-th CPU of the caller's set (the caller's own
+   * mask is restored afterwards: thread 0 is the calling thread) */
+  cpu_set_t caller;
+  int have_mask = g_pin && pthread_getaffinity_np(pthread_self(), sizeof(caller), &caller) == 0;
+  int32_t ncpu = have_mask ? CPU_COUNT(&caller) : 0;
+  for (int32_t t = 0, c = -1, seen = 0; t < threads; t++) {
     args[t].sh = &s;
     args[t].tid = t;
+    args[t].cpu = -1;
+    if (ncpu > 0) {
+      /* the (t mod ncpu)-th CPU present in the caller's mask */
+      int32_t want = t % ncpu;
+      if (want < seen) { c = -1; seen = 0; }
+      while (seen <= want) {
+        c++;
+        if (CPU_ISSET(c, &caller)) seen++;
+      }
+      args[t].cpu = c;
+    }
     if (t > 0) pthread_create(&th[t], NULL, th_frontier, &args[t]);
   }
   th_frontier(&args[0]);
   for (int32_t t = 1; t < threads; t++) pthread_join(th[t], NULL);
+  if (have_mask) pthread_setaffinity_np(pthread_self(), sizeof(caller), &caller);
   pthread_barrier_destroy(&s.bar);
   free(s.q[0]); free(s.q[1]); free(s.stamp); free(args); free(th);
   return 0;

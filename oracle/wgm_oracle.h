@@ -47,6 +47,9 @@ void or_matmul_f64(const float* a, const float* b, double* c, int32_t m, int32_t
 int32_t or_reduce_min(const int32_t* buf, int64_t len);
 /* Multi-threaded frontier BFS / Bellman-Ford (threads play workgroups,
  * one barrier per level); the CPU baseline. */
+/* Pin the threaded restatement's threads (thread t on the t-th CPU of the
+ * caller's affinity set) when on != 0; off by default. */
+int or_threads_pin(int32_t on);
 int or_bfs_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, int32_t src, uint32_t* dist,
                    int32_t threads);
 int or_sssp_threads(int32_t n, const int32_t* row_ptr, const int32_t* col, const int32_t* wt, int32_t src,

@@ -132,3 +132,31 @@ def test_batched_job_times_work_and_order(rt, small_batch):
         assert t0.shape == (len(graphs),) and np.all(t0 >= 0) and np.all(t1 >= t0)
         assert t0.min() == 0.0
     b.close()
+
+
+@pytest.mark.parametrize("wg", [256, 1024])
+def test_batched_standalone_baseline(rt, small_batch, wg):
+    """The per-graph baseline on wgm_bfs / wgm_sssp's standalone kernel:
+    one launch per query on the batch's arrays, every query exact."""
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    st = b.run_standalone(wg_size=wg)
+    assert st["launches"] == len(graphs)
+    _check_all(b, graphs, refs)
+    # the batched launch afterwards is unaffected by the baseline's state
+    b.run(wg_size=256, wgs_per_cu=2, team_edges=512, max_team=8, delta=64)
+    _check_all(b, graphs, refs)
+    b.close()
+
+
+@pytest.mark.parametrize("delta", [64, 512])
+def test_batched_far_flags_in_lds(rt, small_batch, delta):
+    """Low-degree SSSP queries on one 1024-thread workgroup with their near-far
+    flags in the LDS bitmap: exact, with narrow and wide bands."""
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    for flags in (1, 0, 1):
+        b.run(wg_size=1024, wgs_per_cu=1, team_edges=512, max_team=8, delta=delta, delta_high_degree=8,
+              far_flags_lds=flags)
+        _check_all(b, graphs, refs)
+    b.close()

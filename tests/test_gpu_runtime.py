@@ -36,7 +36,7 @@ def test_discovery_grid_within_bound_finds_all(rt):
     assert rt.discover_occupancy(L.K_BARRIER_BENCH, 256, grid) == grid
 
 
-@pytest.mark.parametrize("variant", [L.BARRIER_FLAGS, L.BARRIER_SENSE, L.BARRIER_XCD])
+@pytest.mark.parametrize("variant", [L.BARRIER_FLAGS, L.BARRIER_SENSE, L.BARRIER_XCD, L.BARRIER_COOP])
 @pytest.mark.parametrize("wgs", [2, 64, 0])
 def test_barrier_epochs(rt, variant, wgs):
     r = rt.barrier_bench(wg_size=256, num_wgs=wgs, iterations=20, variant=variant)

@@ -99,3 +99,66 @@ def test_run_merged_driver(rt):
     assert st["np_errors"] == 0
     r, c = np.divmod(np.arange(g.n), 128)
     assert np.array_equal(dist, (r + c).astype(np.uint32))
+
+
+def _topo_ref(g, kind, iterations=50):
+    if kind == S.PERSIST_COLOR:
+        return oracle.color(g.row_ptr, g.col, g.prio)
+    if kind == S.PERSIST_MIS:
+        return oracle.mis(g.row_ptr, g.col, g.prio)
+    return oracle.pagerank_f32(g.row_ptr, g.col, np.float32(0.85), iterations), iterations
+
+
+@pytest.mark.parametrize("query", [False, True])
+@pytest.mark.parametrize("kind", [S.PERSIST_COLOR, S.PERSIST_MIS, S.PERSIST_PAGERANK])
+def test_mega_topology_tasks_with_resizing(rt, kind, query):
+    """Colour, MIS and PageRank as the mega-kernel's persistent task while
+    non-persistent tasks take workgroups away (offer_kill / the query
+    barrier's shrink) and give them back (offer_fork) at every round's or
+    iteration's resizing barriers: colours and sets bit-exact, ranks bitwise
+    equal to the FP32 oracle, the round count equal to the oracle's."""
+    # sized so that the task spans many milliseconds on 15 workgroups
+    side, iters = (256, 400) if kind == S.PERSIST_PAGERANK else (768, 50)
+    g = G.add_priorities(G.grid(side, side))
+    ref, rounds = _topo_ref(g, kind, iters)
+    dg = rt.graph(g)
+    with S.Mega(rt, np_kind=S.NP_REDUCE_MIN) as m:
+        p = m.launch(wg_size=256, num_wgs=16, query_barrier=query)
+        m.send_persistent(dg, kind, 0, p, iterations=iters)
+        n_tasks = 0
+        # every workgroup holds the persistent task, so each task is served by
+        # kills; sizes cycle 4..11
+        while m.persistent_running() and n_tasks < 400:
+            m.send_task(min(4 + n_tasks % 8, p - 1))
+            n_tasks += 1
+        m.wait_persistent(60)
+        m.quit()
+        st = m.stats()
+        out, levels = m.result(g.n)
+    print(kind, query, n_tasks, st)
+    assert st["errors"] == 0 and n_tasks >= 2
+    if kind == S.PERSIST_PAGERANK:
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), f"{int((out != ref).sum())} differ"
+    else:
+        assert np.array_equal(out, ref), f"{int((out != ref).sum())} differ"
+    assert levels == rounds
+
+
+@pytest.mark.parametrize("kind", [S.PERSIST_COLOR, S.PERSIST_MIS, S.PERSIST_PAGERANK])
+def test_mega_topology_tasks_alone(rt, kind):
+    """The same tasks with every workgroup and no interference, on a graph
+    with degree-0 and degree-1 vertices (the colour round's degree-1 rule)."""
+    g = G.add_priorities(G.erdos_renyi(6000, 5000, seed=9))
+    ref, rounds = _topo_ref(g, kind)
+    dg = rt.graph(g)
+    with S.Mega(rt, np_kind=S.NP_REDUCE_MIN) as m:
+        p = m.launch(wg_size=1024, num_wgs=0)
+        m.send_persistent(dg, kind, 0, p)
+        m.wait_persistent(60)
+        m.quit()
+        out, levels = m.result(g.n)
+    if kind == S.PERSIST_PAGERANK:
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    else:
+        assert np.array_equal(out, ref)
+    assert levels == rounds

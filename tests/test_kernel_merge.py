@@ -222,11 +222,38 @@ def test_emitted_kernels_compile_for_gfx950(tmp_path, name):
     assert r.returncode == 0, r.stderr[-3000:]
 
 
+def test_helper_types_and_functions_are_kept(tmp_path):
+    """Code outside the kernel -- the task type and the pool helpers of the
+    octree example -- goes into both emitted forms ahead of the kernel (and
+    of the restoration context, which may hold such types), which then compile
+    for gfx950; comment-only surroundings add nothing."""
+    merged, _ = km.merge(read("reduction.hip"), "MY_reduce", read("octree.hip"), "octree", "mega_octree")
+    alone = km.standalone(read("octree.hip"), "octree")
+    for text, kernel in ((merged, "__device__ void octree("), (alone, "__global__ void octree(")):
+        assert "struct OctTask {" in text and "__device__ inline bool oct_pop(" in text
+        assert text.index("struct OctTask {") < text.index(kernel)
+    assert merged.index("struct OctTask {") < merged.index("struct Restoration_ctx")
+    assert merged.count("struct OctTask {") == 1
+    # the kernel itself appears once, rewritten, not copied into the support code
+    assert alone.count("void octree(") == 1
+    for name, text in (("m.hip", merged), ("s.hip", alone)):
+        src = tmp_path / name
+        src.write_text(text)
+        cmd = ["/opt/rocm/bin/hipcc", "-O3", "-std=c++17", "--offload-arch=gfx950", "--cuda-device-only", "-c",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "gpu_workgroup_managment_amd", "csrc"),
+               "-o", str(tmp_path / (name + ".o")), str(src)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+    assert "struct" not in km.standalone(read("simple_barrier.hip"), "simple_barrier").split("__global__")[0]
+
+
 def test_fixture_library_exports():
     path = os.path.join(KD, "libwgm_kmerge.so")
     if not os.path.exists(path):
         pytest.fail(f"{path} missing: build it with __graft_entry__.build()")
     lib = ctypes.CDLL(path)
-    for f in ("km_mega_barrier", "km_mega_chunks", "km_standalone_barrier", "km_standalone_chunks"):
+    for f in ("km_mega_barrier", "km_mega_chunks", "km_standalone_barrier", "km_standalone_chunks", "km_mega_bfs",
+              "km_standalone_bfs", "km_mega_octree", "km_standalone_octree"):
         getattr(lib, f).restype = ctypes.c_void_p
         assert getattr(lib, f)() != 0
