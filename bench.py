@@ -26,9 +26,15 @@ HBM_PEAK_GBS = 8000.0
 # team granularity 2^14..2^22 edges, chunk 32..4096, SSSP near-far band
 # 0..16384 on the grids and 4..1024 on the RMAT graphs): 1024 x 2 per CU
 # (one fits), teams of ceil(m / 2^20), band 512 on graphs of out-degree <= 4
-# and 8 on the others.
+# and 8 on the others; the single-workgroup SSSP keeps its far flags in an LDS
+# bitmap (7.77 vs 8.56 ms per step with them in global memory).
 TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
-             delta_high_degree=8)
+             delta_high_degree=8, far_flags_lds=1)
+
+
+def log(msg):
+    """One progress line on stderr per finished leg."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -51,6 +57,7 @@ def parse():
     ap.add_argument("--shapes", default="", help="with --suite apps: launch shapes WGSIZExNUMWGS,... (0: occupancy)")
     ap.add_argument("--reps", type=int, default=5, help="with --suite apps: timed launches per shape")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-relaunch", action="store_true", help="skip the one-launch-per-graph baselines")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configured workloads (ER BFS, RMAT-20 SSSP, grid colour/MIS/PageRank, barrier)")
     ap.add_argument("--serving-s", type=float, default=0.5, help="arrival window per offered load")
@@ -240,6 +247,7 @@ def configs_leg(rt, reps=5):
                           "frac": round(ach / HBM_PEAK_GBS, 5)},
              "cpu_oracle_ms": round(cpu_s * 1e3, 2)}
         e.update(extra)
+        log(f"config {extra.get('workload', '')}: {kernel_ms:.3f} ms, bit-exact {bool(exact)}")
         return e
 
     oracle.threads_pin(True)
@@ -263,13 +271,18 @@ def configs_leg(rt, reps=5):
     t0 = time.perf_counter()
     ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
     cpu = time.perf_counter() - t0
-    k, (res, st) = timed(lambda: dg.sssp(0, wg_size=1024, num_wgs=64))
+    # default direction on this graph: pull sweeps over the in-edges; push (frontier near-far) beside it
+    k, (res, st) = timed(lambda: dg.sssp(0, wg_size=1024, num_wgs=0))
+    kp, (resp, stp) = timed(lambda: dg.sssp(0, wg_size=1024, num_wgs=64, mode=1))
     reached = ref != oracle.INF
     ms = int(g.degrees()[reached].sum())
     out["rmat20_sssp"] = entry(k, 12 * ms + 20 * int(reached.sum()), np.array_equal(res, ref), cpu, {
-        "workload": "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", "levels": st["iterations"],
-        "gteps": round(ms / (k * 1e6), 4), "edges_relaxed": st["edges_scanned"],
-        "launch": {"wg_size": 1024, "num_wgs": 64}, "cpu_threads": threads})
+        "workload": "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0",
+        "direction": "pull (Bellman-Ford sweeps over the in-edges)", "sweeps": st["iterations"],
+        "gteps": round(ms / (k * 1e6), 4), "launch": {"wg_size": 1024, "num_wgs": st["participating"]},
+        "push": {"kernel_ms": round(kp, 4), "levels": stp["iterations"], "gteps": round(ms / (kp * 1e6), 4),
+                 "bitexact": bool(np.array_equal(resp, ref)), "launch": {"wg_size": 1024, "num_wgs": 64}},
+        "cpu_threads": threads})
     dg.close()
     oracle.threads_pin(False)
     # 3. colour / MIS / PageRank, grid 2048^2
@@ -312,6 +325,7 @@ def configs_leg(rt, reps=5):
             bar[f"{name}_wg{wg}"] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
                                      "epoch_errors": r["epoch_errors"]}
     out["barrier_full_occupancy"] = bar
+    log("config barrier: done")
     return out
 
 
@@ -382,15 +396,17 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
         t0 = time.perf_counter()
         ref = oracle.sssp_threads(g.row_ptr, g.col, g.wt, 0, threads)
         cpu = time.perf_counter() - t0
-        best = timed(lambda **lk: dg.sssp(0, **lk))
-        res = best[4][0]
         reached = ref != oracle.INF
         ms = int(g.degrees()[reached].sum())
-        line("sssp", "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", best,
-             12 * ms + 20 * int(reached.sum()), bool(np.array_equal(res, ref)),
-             {"edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
-              "relaxations_scanned": best[4][1]["edges_scanned"]},
-             cpu, f"threaded frontier Bellman-Ford, {threads} threads")
+        from gpu_workgroup_managment_amd import _lib as L
+        for app, mode, how in (("sssp", L.SSSP_PUSH, "push: near-far frontier Bellman-Ford"),
+                               ("sssp_pull", L.SSSP_PULL, "pull: Bellman-Ford sweeps over the in-edges")):
+            best = timed(lambda **lk: dg.sssp(0, mode=mode, **lk))
+            line(app, "RMAT scale 20 ef 16 (a,b,c)=(.57,.19,.19), w in [1,255], source 0", best,
+                 12 * ms + 20 * int(reached.sum()), bool(np.array_equal(best[4][0], ref)),
+                 {"direction": how, "edges_scanned": ms, "edges": g.m, "gteps": round(ms / (best[0] * 1e6), 4),
+                  "relaxations_scanned": best[4][1]["edges_scanned"]},
+                 cpu, f"threaded frontier Bellman-Ford, {threads} threads")
         # Pannotia SSSP (min-plus sweeps over in-edges) on the same graph
         t0 = time.perf_counter()
         mref, jacobi = oracle.sssp_minplus(g.row_ptr, g.col, g.wt, 0)
@@ -494,6 +510,7 @@ def main():
         keep = [i for i, it in enumerate(items) if it.kind == kind and it.query == query]
         graphs, queries = [graphs[i] for i in keep], queries[keep]
     gen_s = time.time() - t0
+    log(f"workload: {len(graphs)} graphs in {gen_s:.1f} s")
     rt = wgm.Runtime(0)
     copy_cal = None
     if args.calibrate_copy:
@@ -520,6 +537,7 @@ def main():
     ms_per_step = (t_end - t_start) * 1e3 / args.steps
     edges = st["edges_scanned"]
     gteps = edges / (ms_per_step * 1e6)
+    log(f"batched: {ms_per_step:.3f} ms per step, {gteps:.2f} GTEPS")
 
     # Baseline: one persistent launch per graph, each graph's launch as wide
     # as its team. The best of three team granularities, each with the
@@ -527,20 +545,23 @@ def main():
     # batched shape's coarse teams would understate it).
     relaunch, relaunch_tried = None, []
     shapes = (dict(cfg_d), dict(cfg_d, team_edges=1 << 16, max_team=16), dict(cfg_d, team_edges=1 << 14, max_team=64))
-    for shape in shapes + tuple(dict(sh, delta=1024, delta_high_degree=0) for sh in shapes):
+    for shape in () if args.no_relaunch else shapes + tuple(dict(sh, delta=1024, delta_high_degree=0)
+                                                            for sh in shapes):
         for _ in range(args.relaunch_steps):
             r = batch.run_relaunch(B.config(**shape))
             r["shape"] = {k: shape.get(k, 0) for k in ("team_edges", "max_team", "delta", "delta_high_degree")}
             relaunch_tried.append({"gteps": round(r["gteps"], 4), **r["shape"]})
             relaunch = r if relaunch is None or r["wall_ms"] < relaunch["wall_ms"] else relaunch
+            log(f"relaunch {r['shape']}: {r['gteps']:.3f} GTEPS")
     # baseline 2: each query on the standalone app kernel (wgm_bfs / wgm_sssp), one launch per graph
     standalone = None
-    for wg in (1024, 256):
+    for wg in () if args.no_relaunch else (1024, 256):
         r = batch.run_standalone(wg_size=wg)
         relaunch_tried.append({"gteps": round(r["gteps"], 4), "kernel": "standalone wgm_bfs/wgm_sssp", "wg_size": wg})
         r["shape"] = {"kernel": "standalone wgm_bfs/wgm_sssp", "wg_size": wg}
         standalone = r if standalone is None or r["wall_ms"] < standalone["wall_ms"] else standalone
         relaunch = r if r["wall_ms"] < relaunch["wall_ms"] else relaunch
+        log(f"standalone wg {wg}: {r['gteps']:.3f} GTEPS")
     # the baselines overwrote results: run the batched launch once more for the checks below
     batch.run(cfg)
 
@@ -559,6 +580,7 @@ def main():
     base, exact, sampled = (None, None, 0)
     if not args.no_cpu_baseline:
         base, exact, sampled = cpu_baseline(graphs, queries, batch, args.cpu_budget_s)
+        log(f"cpu baseline: {base['value']} GTEPS, bit-exact {exact}")
     info = rt.device_info()
     serving = None
     p99_at_half = None
@@ -568,6 +590,7 @@ def main():
         batch.close()
         serving = serving_sweep(rt, graphs, queries, digests, (0.25, 0.5, 0.75), args.serving_s)
         p99_at_half = next(x["task_p99_us"] for x in serving["loads"] if x["offered_load"] == 0.5)
+        log(f"serving: task p99 at load 0.5 {p99_at_half} us")
     else:
         batch.close()
     configs = None if args.no_configs else configs_leg(rt)

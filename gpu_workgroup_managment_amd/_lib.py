@@ -40,6 +40,7 @@ BIG_NUM = 99999999  # WGM_BIG_NUM: Pannotia SSSP's unreached distance
 GRAPH_RMAT, GRAPH_GRID = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
 BARRIER_FLAGS, BARRIER_SENSE, BARRIER_XCD, BARRIER_COOP = 0, 1, 2, 3
+SSSP_AUTO, SSSP_PUSH, SSSP_PULL = 0, 1, 2
 
 
 class WgmError(RuntimeError):
@@ -101,7 +102,7 @@ class Launch(ctypes.Structure):
         ("wg_size", ctypes.c_int32),
         ("num_wgs", ctypes.c_int32),
         ("handshake", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
     ]
 
 

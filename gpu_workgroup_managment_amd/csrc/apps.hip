@@ -433,6 +433,17 @@ static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_la
   if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
   if (src < 0 || src >= g->n) return fail(rt, WGM_ERR_INVALID, "source %d out of range", src);
   if (sssp && !g->wt) return fail(rt, WGM_ERR_INVALID, "SSSP needs edge weights");
+  const int mode = lc ? lc->mode : WGM_SSSP_AUTO;
+  if (mode < WGM_SSSP_AUTO || mode > WGM_SSSP_PULL) return fail(rt, WGM_ERR_INVALID, "unknown SSSP mode %d", mode);
+  if (sssp && (mode == WGM_SSSP_PULL || (mode == WGM_SSSP_AUTO && g->maxdeg > 4))) {
+    wgm_run_stats local{};
+    int rc = wgm_sssp_pull(rt, g, src, lc, &local);
+    if (!rc) rc = wgm_graph_reached_edges(g, &local.edges_scanned);
+    if (rc) return rc;
+    if (dist_out) HIPCHK(rt, hipMemcpy(dist_out, g->dist, sizeof(uint32_t) * (size_t)g->n, hipMemcpyDeviceToHost));
+    if (st) *st = local;
+    return WGM_OK;
+  }
   FrontierDev s{};
   NearFar nf{};
   int rc = team_frontier_state(g, sssp, &s, &nf);

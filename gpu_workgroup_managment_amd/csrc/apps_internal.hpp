@@ -69,4 +69,5 @@ int wgm_launch_frontier(wgm_runtime* rt, const wgm::GraphDev& g, wgm::FrontierDe
 int wgm_enqueue_frontier(wgm_runtime* rt, const wgm::GraphDev& g, wgm::FrontierDev s, const wgm::NearFar& nf, bool sssp,
                          int src, int grid, int wg, int* iters_dev);
 uint32_t wgm_standalone_delta(int maxdeg);
+int wgm_sssp_pull(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, wgm_run_stats* st);
 int wgm_enqueue_reached_edges(wgm_runtime* rt, const wgm::GraphDev& g, const uint32_t* dist, unsigned long long* out);

@@ -1166,6 +1166,10 @@ int wgm_batch_run_standalone(wgm_runtime* rt, wgm_batch* b, const wgm_launch* lc
     HIPCHK(rt, hipMalloc(&b->sa_heavy, (3 * hcap + 3 * kLine) * sizeof(int32_t)));
     b->sa_hcap = hcap;
   }
+  // a discovery probe above leaves im_exit set in the control block the
+  // kernels receive by value
+  rt->ctl.im_exit = 0;
+  rt->ctl.handshake = 0;
   const BatchDev d = dev_view(b);
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, rt->stream, d);
   HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));

@@ -92,9 +92,12 @@ __global__ void __launch_bounds__(1024) k_minplus(Ctl c, MinplusDev p, int* iter
   standalone_retire(c);
 }
 
-__global__ void k_minplus_init(int32_t* dist, int n, int src) {
+// Unreached vertices start at `big` (BIG_NUM for Pannotia SSSP, MYINFINITY
+// for Lonestar SSSP's pull direction): big + 255 stays below INT32_MAX, so a
+// candidate through an unreached vertex never lowers anything.
+__global__ void k_minplus_init(int32_t* dist, int n, int src, int32_t big) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    dist[v] = v == src ? 0 : kBigNum;
+    dist[v] = v == src ? 0 : big;
 }
 
 struct BcDev {
@@ -369,7 +372,7 @@ int wgm_sssp_minplus(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launc
   int rc = graph_reverse(g);
   if (rc) return rc;
   if (!g->i32a) HIPCHK(rt, hipMalloc((void**)&g->i32a, sizeof(int32_t) * (size_t)g->n));
-  hipLaunchKernelGGL(k_minplus_init, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, src);
+  hipLaunchKernelGGL(k_minplus_init, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, src, kBigNum);
   MinplusDev p{(const int4*)g->tasks, g->ntasks, g->n + 1, g->rcol, g->rwt, g->i32a};
   Ctl ctl{};
   void* args[] = {&ctl, &p, &g->iters};
@@ -386,6 +389,31 @@ int wgm_sssp_minplus(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launc
   if (st) *st = local;
   return WGM_OK;
 }
+
+}  // extern "C"
+
+// Lonestar SSSP in the pull direction: the min-plus sweeps above on the
+// graph's own distance array with MYINFINITY for unreached vertices, so the
+// result is wgm_sssp's uint32 distances.
+int wgm_sssp_pull(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, wgm_run_stats* st) {
+  int rc = graph_reverse(g);
+  if (rc) return rc;
+  if (!g->dist) HIPCHK(rt, hipMalloc((void**)&g->dist, sizeof(uint32_t) * (size_t)g->n));
+  hipLaunchKernelGGL(k_minplus_init, dim3(256), dim3(256), 0, rt->stream, (int32_t*)g->dist, g->n, src,
+                     (int32_t)kInf);
+  MinplusDev p{(const int4*)g->tasks, g->ntasks, g->n + 1, g->rcol, g->rwt, (int32_t*)g->dist};
+  Ctl ctl{};
+  void* args[] = {&ctl, &p, &g->iters};
+  rc = launch_standalone_app(rt, WGM_K_MINPLUS, minplus_kernel(minplus_lanes(g)), lc, &ctl, args, 0, st);
+  if (rc) return rc;
+  int iters = 0;
+  HIPCHK(rt, hipMemcpy(&iters, g->iters, sizeof(int), hipMemcpyDeviceToHost));
+  if (iters < 0) return fail(rt, WGM_ERR_INVALID, "distances still fall after %d sweeps", g->n + 1);
+  st->iterations = iters;
+  return WGM_OK;
+}
+
+extern "C" {
 
 int wgm_bc(wgm_runtime* rt, wgm_graph* g, int32_t num_sources, const wgm_launch* lc, double* bc_out,
            wgm_run_stats* st) {

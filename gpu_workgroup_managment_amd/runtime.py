@@ -96,11 +96,12 @@ class Runtime:
         return st.as_dict()
 
 
-def _launch(wg_size=256, num_wgs=0, handshake=False):
+def _launch(wg_size=256, num_wgs=0, handshake=False, mode=0):
     lc = L.Launch()
     lc.wg_size = wg_size
     lc.num_wgs = num_wgs
     lc.handshake = 1 if handshake else 0
+    lc.mode = mode
     return lc
 
 

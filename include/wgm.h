@@ -230,8 +230,19 @@ typedef struct wgm_launch {
   int32_t wg_size;     /* threads per workgroup (64..1024, multiple of 64) */
   int32_t num_wgs;     /* <= 0: discovered occupancy */
   int32_t handshake;   /* 1: host flag handshake as in the reference's run_persistent */
-  int32_t reserved;
+  int32_t mode;        /* wgm_sssp's relaxation direction: WGM_SSSP_AUTO, _PUSH or _PULL */
 } wgm_launch;
+
+/* wgm_sssp relaxes in one of two directions (the same distances either way):
+ * push: frontier Bellman-Ford in near-far bands, each frontier vertex's
+ *   out-edges relaxed with atomic minimums (the reference's drelax2);
+ * pull: Bellman-Ford sweeps over the in-edges, each vertex taking the
+ *   minimum over its in-neighbours, until a sweep changes nothing.
+ * AUTO pulls on graphs with an out-degree above 4, whose frontiers reach
+ * most edges within a few levels, and pushes on the others. */
+#define WGM_SSSP_AUTO 0
+#define WGM_SSSP_PUSH 1
+#define WGM_SSSP_PULL 2
 
 typedef struct wgm_run_stats {
   int32_t participating;   /* discovered workgroups */
@@ -248,9 +259,10 @@ typedef struct wgm_run_stats {
  * wgm_bfs: Lonestar BFS (src/lonestar/graph_apps/bfs/bfs.h:16-370,
  *   src/lonestar/graph_apps/bfs/device/standalone.cl:218-320);
  * wgm_sssp: Lonestar SSSP (src/lonestar/graph_apps/sssp/sssp.h:16-382,
- *   src/lonestar/graph_apps/sssp/device/standalone.cl), here in near-far
- *   bands of width 16 on graphs with an out-degree above 4 and 1024 on the
- *   others (same distances; the reference relaxes in plain frontier order);
+ *   src/lonestar/graph_apps/sssp/device/standalone.cl), same distances: by
+ *   default pull sweeps over the in-edges on graphs with an out-degree above
+ *   4, and on the others push in near-far bands of width 1024 (lc->mode);
+ *   st->edges_scanned counts the reached vertices' out-edges either way;
  * wgm_color: Pannotia color (src/pannotia/graph_apps/color/color.h:41-195,
  *   src/pannotia/graph_apps/color/device/standalone.cl:10-126);
  * wgm_mis: Pannotia MIS (src/pannotia/graph_apps/mis/mis.h:40-202,

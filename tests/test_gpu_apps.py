@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
+from gpu_workgroup_managment_amd import _lib as L
 from gpu_workgroup_managment_amd import graphs as G
 
 pytestmark = pytest.mark.gpu
@@ -50,20 +51,26 @@ def test_bfs_handshake(rt):
     assert st["host_ms"] > 0
 
 
+@pytest.mark.parametrize("mode", [L.SSSP_AUTO, L.SSSP_PUSH, L.SSSP_PULL])
 @pytest.mark.parametrize("scale,wgs", [(10, 8), (14, 0)])
-def test_sssp_rmat(rt, scale, wgs):
+def test_sssp_rmat(rt, scale, wgs, mode):
+    """Both relaxation directions give Dijkstra's distances (AUTO pulls here:
+    the out-degree is above 4), and the scanned-edge count of the reached
+    vertices either way."""
     g = G.add_weights(G.rmat(scale, 16, seed=2), 1, 255, 3)
-    dist, st = rt.graph(g).sssp(0, wg_size=256, num_wgs=wgs)
+    dist, st = rt.graph(g).sssp(0, wg_size=256, num_wgs=wgs, mode=mode)
     ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
     assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())
     print(st)
 
 
-def test_sssp_grid_unit_weights(rt):
+@pytest.mark.parametrize("mode", [L.SSSP_AUTO, L.SSSP_PULL])
+def test_sssp_grid_unit_weights(rt, mode):
     side = 128
     g = G.grid(side, side)
     g.wt = np.ones(g.m, np.int32)
-    dist, _ = rt.graph(g).sssp(0, wg_size=256, num_wgs=16)
+    dist, st = rt.graph(g).sssp(0, wg_size=256, num_wgs=16, mode=mode)
     r, c = np.divmod(np.arange(side * side), side)
     assert np.array_equal(dist, (r + c).astype(np.uint32))
 
@@ -158,7 +165,7 @@ def test_frontier_heavy_entries(rt, wg):
     assert st["iterations"] == levels
     g = _hub_chain(wmax=20000)
     dg = rt.graph(g)
-    dist, st = dg.sssp(0, wg_size=wg, num_wgs=16)
+    dist, st = dg.sssp(0, wg_size=wg, num_wgs=16, mode=L.SSSP_PUSH)
     ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
     assert np.array_equal(dist, ref), _first_diff(dist, ref)
     assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())

@@ -82,10 +82,10 @@ def test_config_rmat20_sssp(rt, rmat20):
     g, dg = rmat20
     ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)  # Dijkstra, 64-bit sums
     reached = ref != oracle.INF
-    for wg, nw in ((1024, 64), (256, 0)):
-        dist, st = dg.sssp(0, wg_size=wg, num_wgs=nw)
+    for wg, nw, mode in ((1024, 64, 0), (1024, 0, 0), (1024, 64, 1), (256, 0, 1)):
+        dist, st = dg.sssp(0, wg_size=wg, num_wgs=nw, mode=mode)  # mode 0: pull here; 1: push
         assert np.array_equal(dist, ref), _first_diff(dist, ref)
-        assert st["edges_scanned"] >= int(g.degrees()[reached].sum())
+        assert st["edges_scanned"] == int(g.degrees()[reached].sum())
 
 
 def test_config_rmat20_bfs(rt, rmat20):

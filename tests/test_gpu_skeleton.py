@@ -49,7 +49,7 @@ def km():
 def _bfs_app(km, tmp_path, g, solution=None):
     gr = str(tmp_path / "graph.gr")
     sol = str(tmp_path / "graph.sol")
-    F.write_graph(gr, g, L.FMT_GR)
+    F.write_graph(gr, g, "gr")
     ref, levels = oracle.bfs(g.row_ptr, g.col, 0)
     F.write_solution(sol, ref.astype(np.int32) if solution is None else solution, "colon")
     app = L.PersistentApp()
