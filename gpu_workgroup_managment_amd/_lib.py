@@ -302,6 +302,7 @@ SIGNATURES = {
     "wgm_mis": (_i32, [_vp, _vp, _P(Launch), _vp, _P(RunStats)]),
     "wgm_pagerank": (_i32, [_vp, _vp, ctypes.c_float, _i32, _P(Launch), _vp, _P(RunStats)]),
     "wgm_sssp_minplus": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
+    "wgm_bfs_noalloc": (_i32, [_vp, _vp, _vp, _i32, _i64, _i32, _P(Launch), _vp, _P(RunStats)]),
     "wgm_bc": (_i32, [_vp, _vp, _i32, _P(Launch), _vp, _P(RunStats)]),
     "wgm_mega_create": (_i32, [_vp, _P(NpConfig), _P(_vp)]),
     "wgm_mega_destroy": (_i32, [_vp]),

@@ -370,6 +370,9 @@ int wgm_create(int32_t device, wgm_runtime** out) {
 
   e = hipStreamCreateWithFlags(&rt->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(wgm_hip_fail(nullptr, e, "hipStreamCreate"));
+  // the device address of this GPU's no-allocation scratch array
+  const int nrc = wgm_noalloc_init(rt, device);
+  if (nrc) return bail(nrc);
   e = hipEventCreate(&rt->ev0);
   if (e == hipSuccess) e = hipEventCreate(&rt->ev1);
   if (e != hipSuccess) return bail(wgm_hip_fail(nullptr, e, "hipEventCreate"));

@@ -70,6 +70,8 @@ int wgm_sched_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
 bool wgm_serve_kernel_info(int kernel, KernelInfo* out);
 int wgm_serve_probe(wgm_runtime* rt, int kernel, int grid, int wg_size);
 void wgm_runtime_release_extras(wgm_runtime* rt);
+// defined by the no-allocation unit (noalloc.hip): per-device scratch address
+int wgm_noalloc_init(wgm_runtime* rt, int device);
 
 #define WGM_HIPCHK(rt, call)                                    \
   do {                                                          \

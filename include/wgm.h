@@ -279,6 +279,24 @@ int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
 int wgm_pagerank(wgm_runtime* rt, wgm_graph* g, float damping, int32_t iterations,
                  const wgm_launch* lc, float* rank_out, wgm_run_stats* st);
 
+/* wgm_bfs on a caller-owned device CSR that allocates nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). row_ptr[n + 1], col[m] and dist[n] are
+ * device pointers; the kernel and the results are those of wgm_bfs. The
+ * traversal's working memory (counters, two frontier queues, heavy lists)
+ * is one static device array of WGM_NOALLOC_SCRATCH_BYTES per GPU, so
+ * n <= WGM_NOALLOC_MAX_N and m <= WGM_NOALLOC_MAX_M (WGM_ERR_INVALID beyond).
+ * Every call on a GPU uses that GPU's array and leaves its contents behind:
+ * calls that overlap on one GPU race. lc->num_wgs <= 0 takes the occupancy
+ * this runtime discovered for wgm_bfs at that workgroup size, else the HIP
+ * occupancy bound. */
+#define WGM_NOALLOC_MAX_N (1 << 21)
+#define WGM_NOALLOC_MAX_M (1LL << 26)
+#define WGM_NOALLOC_SCRATCH_BYTES 17565568LL
+int wgm_bfs_noalloc(wgm_runtime* rt, const int32_t* row_ptr, const int32_t* col, int32_t n, int64_t m, int32_t src,
+                    const wgm_launch* lc, uint32_t* dist, wgm_run_stats* st);
+
 /* Pannotia SSSP: run_persistent with the sssp plugin
  * (src/pannotia/graph_apps/sssp/sssp.h:40-176, kernel sssp_combined,
  * src/pannotia/graph_apps/sssp/device/sssp.cl:11-92). Min-plus sweeps over the
