@@ -27,9 +27,10 @@ HBM_PEAK_GBS = 8000.0
 # 0..16384 on the grids and 4..1024 on the RMAT graphs): 1024 x 2 per CU
 # (one fits), teams of ceil(m / 2^20), band 512 on graphs of out-degree <= 4
 # and 8 on the others; the single-workgroup SSSP keeps its far flags in an LDS
-# bitmap (7.77 vs 8.56 ms per step with them in global memory).
+# bitmap (7.77 vs 8.56 ms per step with them in global memory), and the
+# single-workgroup low-degree queries read packed rows (7.00 vs 7.77 ms).
 TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
-             delta_high_degree=8, far_flags_lds=1)
+             delta_high_degree=8, far_flags_lds=1, low_ell=1)
 
 
 def log(msg):
@@ -51,6 +52,8 @@ def parse():
                     help="apps: one line per standalone kernel on its workload (not the headline line)")
     ap.add_argument("--far-flags-lds", type=int, default=None, choices=(0, 1),
                     help="override the tuned setting of the LDS far-flag bitmap")
+    ap.add_argument("--low-ell", type=int, default=None, choices=(0, 1),
+                    help="override the tuned setting of the packed rows of low-degree graphs")
     ap.add_argument("--only", choices=("rmat-bfs", "rmat-sssp", "grid-bfs", "grid-sssp"),
                     help="time one class of the batched workload (not the headline line)")
     ap.add_argument("--apps", default="", help="with --suite apps: a subset of bfs,sssp,bc,grid,barrier")
@@ -523,6 +526,8 @@ def main():
         cfg_d = {k: v for k, v in best.as_dict().items() if k not in ("use_graph", "reserved")}
     if args.far_flags_lds is not None:
         cfg_d["far_flags_lds"] = args.far_flags_lds
+    if args.low_ell is not None:
+        cfg_d["low_ell"] = args.low_ell
     cfg = B.config(use_graph=True, **cfg_d)
 
     for _ in range(args.warmup):

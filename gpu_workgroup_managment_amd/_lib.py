@@ -132,6 +132,7 @@ class BatchConfig(ctypes.Structure):
         ("sssp_team_edges", ctypes.c_int32),
         ("delta_high_degree", ctypes.c_int32),
         ("far_flags_lds", ctypes.c_int32),
+        ("low_ell", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -10,12 +10,13 @@ from . import graphs as G
 
 
 def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, use_graph=False, delta=0,
-           sssp_team_edges=0, delta_high_degree=0, far_flags_lds=0):
+           sssp_team_edges=0, delta_high_degree=0, far_flags_lds=0, low_ell=0):
     c = L.BatchConfig()
     c.wg_size, c.wgs_per_cu, c.chunk = wg_size, wgs_per_cu, chunk
     c.team_edges, c.max_team, c.use_graph = team_edges, max_team, 1 if use_graph else 0
     c.delta, c.sssp_team_edges, c.delta_high_degree = delta, sssp_team_edges, delta_high_degree
     c.far_flags_lds = 1 if far_flags_lds else 0
+    c.low_ell = 1 if low_ell else 0
     return c
 
 

@@ -54,6 +54,12 @@ __host__ __device__ constexpr int low_lds_cap(int bd) { return 2 * bd; }
 // setting a far flag is then an LDS atomic instead of a round trip to L2.
 constexpr int kFarBitWords = 8192;
 __host__ __device__ constexpr int batch_far_bits_bytes(int bd) { return bd >= 1024 ? 4 * kFarBitWords : 0; }
+
+// Packed rows of low-degree graphs: kLowDegree words per vertex, (target << 8)
+// | weight, kEllNone past the degree. A graph gets them when every weight fits
+// 8 bits and its vertex ids fit 24 (kEllNone is no valid word then).
+constexpr uint32_t kEllNone = 0xffffffffu;
+constexpr int32_t kEllMaxN = (1 << 24) - 1;
 __host__ __device__ constexpr int batch_lds_bytes(int bd) {
   return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd);
 }
@@ -69,6 +75,7 @@ struct BatchJob {
   int maxdeg;         // largest out-degree (picks the degree-bounded frontier step)
   int qcap;           // entries per queue: n + 1 (BFS), max(n, m) + 1 (SSSP, one entry per relaxation)
   uint32_t delta;     // SSSP near-far band width of this query (0: off)
+  long long e_off;    // into the packed-row pool (kLowDegree words per vertex), -1: none
 };
 
 struct BatchDev {
@@ -97,6 +104,8 @@ struct BatchDev {
   unsigned long long* relaxed;  // per job: edges relaxed from live frontier entries (work)
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
   int far_bits;                 // 1: low-degree single-workgroup SSSP keeps its far flags in the LDS bitmap
+  const uint32_t* ell;          // packed rows of the low-degree graphs (BatchJob::e_off)
+  int low_ell;                  // 1: low-degree single-workgroup queries read the packed rows
 };
 
 
@@ -207,27 +216,31 @@ struct LowQueues {
   lds_u32* lds;  // [queue][vertex, row info, tag][cap]
   int cap;
   const FrontierDev* s;
-  __device__ __forceinline__ void load(int p, int i, int& v, uint32_t& ri, uint32_t& du, bool tag) const {
+  // rows: the entry carries row_info (CSR rows; packed rows need none)
+  __device__ __forceinline__ void load(int p, int i, int& v, uint32_t& ri, uint32_t& du, bool tag,
+                                       bool rows = true) const {
+    ri = 0u;
     if (i < cap) {
       const lds_u32* b = lds + p * 3 * cap;
       v = (int)b[i];
-      ri = b[cap + i];
+      if (rows) ri = b[cap + i];
       du = tag ? b[2 * cap + i] : 0u;
     } else {
       v = (p ? s->q1 : s->q0)[i];
-      ri = (p ? s->qr1 : s->qr0)[i];
+      if (rows) ri = (p ? s->qr1 : s->qr0)[i];
       du = tag ? (p ? s->qa1 : s->qa0)[i] : 0u;
     }
   }
-  __device__ __forceinline__ void store(int p, int i, int v, uint32_t ri, uint32_t du, bool tag) const {
+  __device__ __forceinline__ void store(int p, int i, int v, uint32_t ri, uint32_t du, bool tag,
+                                        bool rows = true) const {
     if (i < cap) {
       lds_u32* b = lds + p * 3 * cap;
       b[i] = (uint32_t)v;
-      b[cap + i] = ri;
+      if (rows) b[cap + i] = ri;
       if (tag) b[2 * cap + i] = du;
     } else {
       (p ? s->q1 : s->q0)[i] = v;
-      (p ? s->qr1 : s->qr0)[i] = ri;
+      if (rows) (p ? s->qr1 : s->qr0)[i] = ri;
       if (tag) (p ? s->qa1 : s->qa0)[i] = du;
     }
   }
@@ -245,9 +258,13 @@ struct LowQueues {
 // bands as in run_single; a far queue of at most bd entries is split in one
 // pass that keeps each entry's distance and row in registers across the
 // band minimum.
-template <bool SSSP, int K, int U>
+// ELL: the graph is read from packed rows instead of the CSR -- K words per
+// vertex, word k = (target << 8) | weight of its k-th edge, kEllNone past its
+// degree -- so the K threads of an entry load one 4K-byte row together, an
+// entry needs no row_info, and the second round trip is the atomic alone.
+template <bool SSSP, int K, int U, bool ELL>
 __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
-                        int* iters_out, bool lds_flags) {
+                        int* iters_out, bool lds_flags, const uint32_t* ell) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
   const LowQueues lq{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
@@ -263,7 +280,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   if (bits)
     for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x) fbits[w] = 0u;
   if (threadIdx.x == 0) {
-    lq.store(0, 0, src, row_info(g, src), 0u, SSSP);
+    lq.store(0, 0, src, ELL ? 0u : row_info(g, src), 0u, SSSP, !ELL);
     s_cnt[0] = 1;
     s_cnt[1] = 0;
     s_cnt[2] = 0;
@@ -307,9 +324,9 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           uint32_t ri = 0;
           v[x] = 0;
           du[x] = 0;
-          if (lv[x] && i < in_n) lq.load(p, i, v[x], ri, du[x], SSSP);
-          ok[x] = (int)(ri & 7u) > k;
-          e[x] = (int)(ri >> 3) + k;
+          if (lv[x] && i < in_n) lq.load(p, i, v[x], ri, du[x], SSSP, !ELL);
+          ok[x] = ELL ? (lv[x] && i < in_n) : (int)(ri & 7u) > k;
+          e[x] = ELL ? v[x] * K + k : (int)(ri >> 3) + k;
         }
         // round trip 1, every slot in flight: the edge and the entry's distance
         int u[U];
@@ -320,9 +337,17 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           w[x] = 0u;
           dv[x] = 0u;
           if (!lv[x]) continue;
-          u[x] = ok[x] ? g.col[e[x]] : 0;
-          w[x] = SSSP && ok[x] ? (uint32_t)g.wt[e[x]] : 0u;
-          dv[x] = SSSP && ok[x] ? s.dist[v[x]] : 0u;
+          if (ELL) {
+            const uint32_t word = ok[x] ? ell[e[x]] : kEllNone;
+            dv[x] = SSSP && ok[x] ? s.dist[v[x]] : 0u;
+            ok[x] = word != kEllNone;
+            u[x] = ok[x] ? (int)(word >> 8) : 0;
+            w[x] = word & 255u;
+          } else {
+            u[x] = ok[x] ? g.col[e[x]] : 0;
+            w[x] = SSSP && ok[x] ? (uint32_t)g.wt[e[x]] : 0u;
+            dv[x] = SSSP && ok[x] ? s.dist[v[x]] : 0u;
+          }
         }
         // round trip 2, every slot in flight: the atomic and the target's row.
         // Only the memory operations sit under each slot's condition; what
@@ -339,10 +364,12 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           r1[x] = 0u;
           old[x] = 0u;  // not live: never an improvement, never kInf
           if (!lv[x]) continue;
-          int uu = live ? u[x] : 0;
-          opaque(uu);  // one unconditional load pair, not one per arm of live
-          r0[x] = (uint32_t)g.row_ptr[uu];
-          r1[x] = (uint32_t)g.row_ptr[uu + 1];
+          if (!ELL) {
+            int uu = live ? u[x] : 0;
+            opaque(uu);  // one unconditional load pair, not one per arm of live
+            r0[x] = (uint32_t)g.row_ptr[uu];
+            r1[x] = (uint32_t)g.row_ptr[uu + 1];
+          }
           if (live) {
             // BFS: every distance set so far is <= level + 1 = alt, so the
             // minimum changes only kInf and old == kInf marks the discovery
@@ -375,7 +402,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             if (lane == leader)
               base = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
-            if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP);
+            if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP, !ELL);
           }
           if (SSSP) {
             bool far_want;
@@ -422,7 +449,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           nb = __hip_atomic_fetch_add(&s_cnt[level % 3], __popcll(mask), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         nb = __builtin_amdgcn_readlane(nb, leader);
-        if (to_near) lq.store(p, nb + __popcll(mask & below), v, rv, dv, true);
+        if (to_near) lq.store(p, nb + __popcll(mask & below), v, rv, dv, true, !ELL);
       }
       push_item<false>(to_far, v, nf.far(1 - fi), &s_far[1 - fi], __HIP_MEMORY_SCOPE_WORKGROUP);
     };
@@ -433,7 +460,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
       if (in) {
         v = nf.far(fi)[threadIdx.x];
         dv = s.dist[v];
-        rv = row_info(g, v);
+        rv = ELL ? 0u : row_info(g, v);
       }
       const int mn = wave_min_int(in && dv >= t_old ? (int)dv : INT32_MAX);
       if (lane == 0) atomicMin(&s_far[2], mn);
@@ -458,7 +485,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         if (in) {
           v = nf.far(fi)[i];
           dv = s.dist[v];
-          rv = row_info(g, v);
+          rv = ELL ? 0u : row_info(g, v);
         }
         split(in, v, dv, rv);
       }
@@ -549,8 +576,15 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       // (row starts must fit row_info's 29 bits)
       if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
         const bool lds_flags = b.far_bits && batch_far_bits_bytes((int)blockDim.x) >= 4 * ((jb.n + 31) / 32);
-        if (sssp) run_low<true, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags);
-        else run_low<false, kLowDegree, kLowSlots>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false);
+        if (b.low_ell && jb.e_off >= 0) {
+          const uint32_t* ell = b.ell + jb.e_off;
+          if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
+          else run_low<false, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false, ell);
+        } else {
+          if (sssp)
+            run_low<true, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, nullptr);
+          else run_low<false, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false, nullptr);
+        }
       } else {
         if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
         else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
@@ -654,8 +688,9 @@ struct wgm_batch {
   wgm_runtime* rt = nullptr;
   int count = 0;
   std::vector<int32_t> n, query, src, maxdeg;
-  std::vector<int64_t> m, row_off, col_off, v_off, q_off;
-  int64_t total_rows = 0, total_edges = 0, total_v = 0, total_q = 0;
+  std::vector<int64_t> m, row_off, col_off, v_off, q_off, e_off;
+  int64_t total_rows = 0, total_edges = 0, total_v = 0, total_q = 0, total_e = 0;
+  uint32_t* ell = nullptr;  // packed rows of the low-degree graphs (e_off, -1: none)
   bool weighted = false;
   int32_t *row = nullptr, *col = nullptr, *wt = nullptr, *q = nullptr, *counts = nullptr, *tbar = nullptr;
   int32_t* claims = nullptr;
@@ -737,6 +772,7 @@ BatchDev dev_view(wgm_batch* b) {
   d.far = b->far;
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
+  d.ell = b->ell;
   return d;
 }
 
@@ -791,6 +827,7 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.maxdeg = b->maxdeg[i];
     j.qcap = b->qcap[i];
     j.delta = (uint32_t)(b->maxdeg[i] > kLowDegree && cfg.delta_high_degree > 0 ? cfg.delta_high_degree : cfg.delta);
+    j.e_off = b->e_off[i];
     ticket += j.team;
   }
   b->chunk = cfg.chunk;
@@ -834,6 +871,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
   d.far_bits = cfg.far_flags_lds ? 1 : 0;
+  d.low_ell = cfg.low_ell ? 1 : 0;
   hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
   return enqueue_collect(b, st);
 }
@@ -947,6 +985,10 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     b->v_off.push_back(b->total_v);
     b->q_off.push_back(b->total_q);
     b->f_off.push_back(b->total_f);
+    bool packed = md <= kLowDegree && g.n <= kEllMaxN;
+    for (int64_t k = 0; packed && g.wt && k < g.m; k++) packed = g.wt[k] >= 0 && g.wt[k] <= 255;
+    b->e_off.push_back(packed ? b->total_e : -1);
+    if (packed) b->total_e += ((int64_t)kLowDegree * g.n + 63) / 64 * 64;
     // SSSP queues carry distance tags and a level queues at most one entry
     // per relaxation; BFS claims each vertex once
     const int64_t cap = (queries[i] == WGM_QUERY_SSSP ? std::max<int64_t>(g.n, g.m) : g.n) + 1;
@@ -983,6 +1025,7 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->far, sizeof(int32_t) * b->total_f);
   if (e == hipSuccess) e = alloc((void**)&b->in_far, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = alloc((void**)&b->fcnt, sizeof(int32_t) * 3 * kLine * count);
+  if (e == hipSuccess) e = alloc((void**)&b->ell, sizeof(uint32_t) * b->total_e);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(batch pools)"));
   // zeroed state: a queue slot read before it is written holds a vertex id of
   // its own job (0 at first), never garbage
@@ -1009,6 +1052,18 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     }
     if (e == hipSuccess && !stage.empty()) e = hipMemcpy(b->wt, stage.data(), sizeof(int32_t) * stage.size(), hipMemcpyHostToDevice);
   }
+  if (b->total_e > 0 && e == hipSuccess) {
+    std::vector<uint32_t> rows((size_t)b->total_e, kEllNone);
+    for (int i = 0; i < count; i++) {
+      if (b->e_off[i] < 0) continue;
+      const wgm_csr& g = graphs[i];
+      uint32_t* r = rows.data() + b->e_off[i];
+      for (int32_t v = 0; v < g.n; v++)
+        for (int32_t k = g.row_ptr[v]; k < g.row_ptr[v + 1]; k++)
+          r[(size_t)v * kLowDegree + (k - g.row_ptr[v])] = ((uint32_t)g.col[k] << 8) | (g.wt ? (uint32_t)g.wt[k] : 0u);
+    }
+    e = hipMemcpy(b->ell, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "batch upload"));
   *out = b;
   return WGM_OK;
@@ -1019,7 +1074,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->res, b->dist, b->qa, b->qr,
-                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->sa_heavy};
+                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->sa_heavy, b->ell};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -1028,7 +1083,7 @@ int wgm_batch_destroy(wgm_batch* b) {
 
 int64_t wgm_batch_bytes(const wgm_batch* b) {
   if (!b) return 0;
-  return 4 * (b->total_rows + b->total_edges * (b->weighted ? 2 : 1));
+  return 4 * (b->total_rows + b->total_edges * (b->weighted ? 2 : 1) + b->total_e);
 }
 
 int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in, wgm_batch_stats* st) {
@@ -1107,6 +1162,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
   for (int k = 0; k < b->count; k++) {
     BatchDev one = d;
     one.far_bits = cfg.far_flags_lds ? 1 : 0;
+    one.low_ell = cfg.low_ell ? 1 : 0;
     one.jobs = b->d_jobs_one + k;  // same job, tickets numbered from 0
     one.njobs = 1;
     one.total_tickets = b->jobs[k].team;

@@ -699,6 +699,9 @@ typedef struct wgm_batch_config {
                                 long narrow traversals of low-degree graphs wide ones (fewer levels) */
   int32_t far_flags_lds;     /* 1: a low-degree SSSP query on one 1024-thread workgroup keeps its near-far
                                 flags in an LDS bitmap (graphs up to 2^18 vertices) instead of HBM */
+  int32_t low_ell;           /* 1: a low-degree query on one workgroup reads its graph from packed rows,
+                                4 words per vertex of (target << 8) | weight, built at create for graphs
+                                of out-degree <= 4, fewer than 2^24 - 1 vertices and weights in [0, 255] */
 } wgm_batch_config;
 
 typedef struct wgm_batch_stats {
@@ -719,7 +722,7 @@ typedef struct wgm_batch_stats {
 int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const int32_t* queries,
                      const int32_t* sources, wgm_batch** out);
 int wgm_batch_destroy(wgm_batch* b);
-int64_t wgm_batch_bytes(const wgm_batch* b);
+int64_t wgm_batch_bytes(const wgm_batch* b); /* resident graph bytes: CSR, weights and packed rows */
 /* One persistent launch serving every query (cfg may be NULL for defaults). */
 int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
 /* Baseline: one persistent launch per graph, back to back. */

@@ -160,3 +160,37 @@ def test_batched_far_flags_in_lds(rt, small_batch, delta):
               far_flags_lds=flags)
         _check_all(b, graphs, refs)
     b.close()
+
+
+@pytest.mark.parametrize("wg,delta", [(1024, 512), (1024, 0), (256, 64)])
+def test_batched_packed_rows(rt, small_batch, wg, delta):
+    """Low-degree queries on one workgroup reading packed rows ((target << 8) |
+    weight, four words per vertex) instead of the CSR: exact for BFS and SSSP,
+    alternating with the CSR step on the same batch."""
+    graphs, queries, refs = small_batch
+    b = B.Batch(rt, graphs, queries)
+    for ell in (1, 0, 1):
+        b.run(wg_size=wg, wgs_per_cu=1, team_edges=512, max_team=8, delta=delta, delta_high_degree=8,
+              far_flags_lds=1, low_ell=ell)
+        _check_all(b, graphs, refs)
+    b.close()
+
+
+def test_batched_packed_rows_skip_wide_weights(rt):
+    """A low-degree graph with a weight above 255 keeps the CSR step (its
+    weights do not fit the packed word); its neighbours in the batch take the
+    packed rows; every query exact."""
+    from gpu_workgroup_managment_amd import graphs as G
+    wide = G.add_weights(G.grid(40, 40), seed=3)
+    wide.wt[7] = 300
+    graphs = [G.add_weights(G.grid(30, 20), seed=1), wide, G.grid(25, 25)]
+    queries = np.array([L.QUERY_SSSP, L.QUERY_SSSP, L.QUERY_BFS], np.int32)
+    refs = [oracle.sssp(graphs[0].row_ptr, graphs[0].col, graphs[0].wt, 0),
+            oracle.sssp(wide.row_ptr, wide.col, wide.wt, 0), oracle.bfs(graphs[2].row_ptr, graphs[2].col, 0)[0]]
+    b = B.Batch(rt, graphs, queries)
+    # packed rows (four words per vertex, 64-word aligned) for the first and last graph only
+    rows = sum((4 * g.n + 63) // 64 * 64 for g in (graphs[0], graphs[2]))
+    assert b.nbytes() == 4 * (sum(g.n + 1 + 2 * g.m for g in graphs) + rows)
+    b.run(wg_size=1024, wgs_per_cu=1, delta=64, far_flags_lds=1, low_ell=1)
+    _check_all(b, graphs, refs)
+    b.close()
