@@ -103,7 +103,7 @@ struct BatchDev {
   unsigned long long* reached;  // per job: out-degrees of the reached vertices, summed after the traversal
   unsigned long long* relaxed;  // per job: edges relaxed from live frontier entries (work)
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
-  int far_bits;                 // 1: low-degree single-workgroup SSSP keeps its far flags in the LDS bitmap
+  int far_bits;                 // 1: low-degree single-workgroup queries keep per-vertex bits in LDS (far / visited)
   const uint32_t* ell;          // packed rows of the low-degree graphs (BatchJob::e_off)
   int low_ell;                  // 1: low-degree single-workgroup queries read the packed rows
 };
@@ -269,16 +269,20 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
   const LowQueues lq{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
                      low_lds_cap((int)blockDim.x), &s};
-  // far flags: one bit per vertex in LDS (lds_flags), else a word per vertex in HBM
+  // one bit per vertex in LDS (lds_flags): SSSP's far flags (else a word per
+  // vertex in HBM); BFS's visited marks, which then replace the atomic on
+  // dist (else the atomic minimum decides the discovery)
   lds_u32* fbits = (lds_u32*)(reinterpret_cast<uint32_t*>(lds) + (frontier_lds_bytes((int)blockDim.x) +
                                                                    2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
   const bool bits = near_far && lds_flags;
+  const bool vis = !SSSP && lds_flags;
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
     if (near_far && !bits) nf.flag[v] = 0u;
   }
-  if (bits)
-    for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x) fbits[w] = 0u;
+  if (bits || vis)
+    for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x)
+      fbits[w] = vis && w == (src >> 5) ? 1u << (src & 31) : 0u;
   if (threadIdx.x == 0) {
     lq.store(0, 0, src, ELL ? 0u : row_info(g, src), 0u, SSSP, !ELL);
     s_cnt[0] = 1;
@@ -371,10 +375,17 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             r1[x] = (uint32_t)g.row_ptr[uu + 1];
           }
           if (live) {
-            // BFS: every distance set so far is <= level + 1 = alt, so the
-            // minimum changes only kInf and old == kInf marks the discovery
-            // (a compare-and-swap without its compare inside the block)
-            old[x] = __hip_atomic_fetch_min(&s.dist[u[x]], alt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (vis) {
+              // BFS with visited bits: the first to set u's bit discovers it
+              const uint32_t bit = 1u << (u[x] & 31);
+              old[x] = (__hip_atomic_fetch_or(&fbits[u[x] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                        bit) ? 0u : kInf;
+            } else {
+              // BFS: every distance set so far is <= level + 1 = alt, so the
+              // minimum changes only kInf and old == kInf marks the discovery
+              // (a compare-and-swap without its compare inside the block)
+              old[x] = __hip_atomic_fetch_min(&s.dist[u[x]], alt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
           }
         }
 #pragma unroll
@@ -391,6 +402,8 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           const bool better = SSSP ? alt[x] < old[x] : old[x] == kInf;
           want[x] = better && (!SSSP || alt[x] < thr);
           far_cand[x] = SSSP && better && alt[x] >= thr;
+          // the discoverer writes the level: nothing reads dist during a BFS
+          if (vis && want[x]) s.dist[u[x]] = alt[x];
         }
 #pragma unroll
         for (int x = 0; x < U; x++) {
@@ -579,11 +592,11 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
         if (b.low_ell && jb.e_off >= 0) {
           const uint32_t* ell = b.ell + jb.e_off;
           if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
-          else run_low<false, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false, ell);
+          else run_low<false, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
         } else {
           if (sssp)
             run_low<true, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, nullptr);
-          else run_low<false, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], false, nullptr);
+          else run_low<false, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, nullptr);
         }
       } else {
         if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);

@@ -697,8 +697,10 @@ typedef struct wgm_batch_config {
   int32_t delta_high_degree; /* SSSP band width on graphs with an out-degree above 4 (0: delta):
                                 their few wide levels favour narrow bands (less rework), the
                                 long narrow traversals of low-degree graphs wide ones (fewer levels) */
-  int32_t far_flags_lds;     /* 1: a low-degree SSSP query on one 1024-thread workgroup keeps its near-far
-                                flags in an LDS bitmap (graphs up to 2^18 vertices) instead of HBM */
+  int32_t far_flags_lds;     /* 1: a low-degree query on one 1024-thread workgroup keeps one bit per vertex
+                                in an LDS bitmap (graphs up to 2^18 vertices): SSSP its near-far flags
+                                (instead of a word per vertex in HBM), BFS its visited marks (instead of
+                                an atomic minimum on the distance per edge) */
   int32_t low_ell;           /* 1: a low-degree query on one workgroup reads its graph from packed rows,
                                 4 words per vertex of (target << 8) | weight, built at create for graphs
                                 of out-degree <= 4, fewer than 2^24 - 1 vertices and weights in [0, 255] */
