@@ -25,12 +25,14 @@ HBM_PEAK_GBS = 8000.0
 # Tuned on MI355X over this workload (wg 64..1024, 1..8 workgroups per CU,
 # team granularity 2^14..2^22 edges, chunk 32..4096, SSSP near-far band
 # 0..16384 on the grids and 4..1024 on the RMAT graphs): 1024 x 2 per CU
-# (one fits), teams of ceil(m / 2^20), band 512 on graphs of out-degree <= 4
-# and 8 on the others; the single-workgroup SSSP keeps its far flags in an LDS
-# bitmap (7.77 vs 8.56 ms per step with them in global memory), and the
-# single-workgroup low-degree queries read packed rows (7.00 vs 7.77 ms).
+# (one fits), BFS teams of ceil(m / 2^20) and SSSP teams of ceil(m / 786432)
+# workgroups, band 512 on graphs of out-degree <= 4 and 12 on the others; the
+# single-workgroup low-degree queries keep per-vertex bits in an LDS bitmap
+# (7.77 vs 8.56 ms per step with the far flags in global memory) and read
+# packed rows (7.00 vs 7.77 ms). The SSSP teams' granularity and band took the
+# step from 6.79 to about 6.1 ms.
 TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
-             delta_high_degree=8, far_flags_lds=1, low_ell=1)
+             delta_high_degree=12, far_flags_lds=1, low_ell=1, sssp_team_edges=786432)
 
 
 def log(msg):

@@ -263,8 +263,14 @@ struct NearFar {
 
 // One level of frontier relaxation over q[level & 1] by a team of nteam
 // workgroups. Static mode: chunks of bd entries round-robin over the team.
-// Dynamic mode (s.chunk > 0): workgroups claim s.chunk entries at a time from
-// a per-level counter, which balances uneven chunks.
+// Dynamic mode (s.chunk > 0): workgroups claim up to s.chunk entries at a
+// time from a per-level counter, which balances uneven chunks. A narrow
+// level is claimed in smaller pieces -- about two per member, at least 64
+// entries -- so that it still spreads over the team: with s.chunk entries per
+// claim, a level of a few thousand entries would keep all but a few members
+// idle while those walk every edge. Members may claim different sizes (a
+// claim takes [c0, c0 + its size) from the counter), so the claimed ranges
+// still tile the level.
 template <bool SSSP, bool COH = false, bool PAIRS = false>
 __device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
                                                int rank, int nteam, int* lds, uint32_t thr = 0xffffffffu,
@@ -282,13 +288,14 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
   if (s.chunk > 0 && s.claims) {
     int* s_claim = lds + 3 * bd + 17;
     int* claim = &s.claims[(level % 3) * kLine];
+    const int chunk = min(s.chunk, max(64, (in_n / (2 * nteam) + 63) & ~63));
     while (true) {
-      if (tid == 0) *s_claim = add_relaxed(claim, s.chunk);
+      if (tid == 0) *s_claim = add_relaxed(claim, chunk);
       __syncthreads();
       const int c0 = *s_claim;
       __syncthreads();
       if (c0 >= in_n) break;
-      const int c1 = min(c0 + s.chunk, in_n);
+      const int c1 = min(c0 + chunk, in_n);
       for (int a = c0; a < c1; a += bd)
         frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr,
                                                 far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);

@@ -59,7 +59,8 @@ __device__ void p_frontier(const SchedDev& s, const MegaApp& a, RCtx rin, int di
   while (true) {
     level++;
     if (ld_relaxed(&a.f.counts[(level % 3) * kLine]) == 0) break;
-    frontier_level<SSSP>(a.g, a.f, level, vid, 2, lds);
+    // the current group count sizes a narrow level's claims
+    frontier_level<SSSP>(a.g, a.f, level, vid, max(1, (int)ld_relaxed(s.bar_groups)), lds);
     if (resize_barrier(s, vid, RCtx{1, level})) return;
   }
   if (vid == 0 && threadIdx.x == 0) *a.iters = level;

@@ -37,7 +37,7 @@ with open(os.path.join(HERE, "golden", "fixtures.json")) as _f:
 THREADS = min(16, os.cpu_count() or 1)
 # the benchmark's launch shape for the 512-graph batch
 BATCH_SHAPE = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
-                   delta_high_degree=8)
+                   delta_high_degree=12, sssp_team_edges=786432)
 
 
 def _first_diff(a, b):
