@@ -64,6 +64,12 @@ __host__ __device__ constexpr int batch_lds_bytes(int bd) {
   return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd);
 }
 
+// The per-vertex bitmap: the last batch_far_bits_bytes(bd) of the dynamic LDS.
+__device__ __forceinline__ lds_u32* batch_bits(int* lds) {
+  return (lds_u32*)(reinterpret_cast<uint32_t*>(lds) +
+                    (frontier_lds_bytes((int)blockDim.x) + 2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
+}
+
 struct BatchJob {
   long long row_off;  // into the row_ptr pool
   long long col_off;  // into the col / wt pools
@@ -114,15 +120,21 @@ struct BatchDev {
 // SSSP with nf.delta > 0 runs near-far bands: levels relax only below the
 // band's threshold; when the near frontier empties, the threshold moves to
 // the smallest deferred distance + delta and the far queue is split.
+// lds_flags: one bit per vertex in the LDS bitmap (batch_bits) -- BFS's
+// visited marks, SSSP's far flags -- instead of per-vertex words in HBM.
 template <bool SSSP>
 __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
-                           int* iters_out) {
+                           int* iters_out, bool lds_flags) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
+  lds_u32* bits = lds_flags && (near_far || !SSSP) ? batch_bits(lds) : nullptr;
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
     s.dist[v] = (v == src) ? 0u : kInf;
-    if (near_far) nf.flag[v] = 0u;
+    if (near_far && !bits) nf.flag[v] = 0u;
   }
+  if (bits)
+    for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x)
+      bits[w] = !SSSP && w == (src >> 5) ? 1u << (src & 31) : 0u;
   if (threadIdx.x == 0) {
     s.q0[0] = src;
     if (SSSP) s.qa0[0] = 0u;
@@ -148,7 +160,7 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
       for (int a = 0; a < in_n; a += bd)
         frontier_batch<SSSP, true, false, SSSP>(g, s, in_q, a, min(a + bd, in_n), out_q, &s_cnt[(level + 1) % 3],
                                                 (uint32_t)level + 1u, lds, thr, near_far ? nf.far(fi) : nullptr,
-                                                &s_far[fi], nf.flag, in_qa, out_qa);
+                                                &s_far[fi], nf.flag, in_qa, out_qa, nullptr, bits);
       if (threadIdx.x == 0) s_cnt[(level + 2) % 3] = 0;
       __syncthreads();
     }
@@ -176,7 +188,10 @@ __device__ void run_single(const GraphDev& g, const FrontierDev& s, const NearFa
         v = nf.far(fi)[i];
         dv = s.dist[v];
         if (dv < thr) {
-          nf.flag[v] = 0u;  // each deferred vertex is in one far queue once: it moves once
+          // each deferred vertex is in one far queue once: it moves once
+          if (bits) __hip_atomic_fetch_and(&bits[v >> 5], ~(1u << (v & 31)), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+          else nf.flag[v] = 0u;
           to_near = dv >= t_old;
         } else {
           to_far = true;  // stays flagged in the other far queue
@@ -207,10 +222,10 @@ __device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
 
 // Frontier queues of the low-degree step: entry i of queue p (0 or 1) in
 // LDS below cap, in the HBM queue at index i above.
-// LDS-typed, so that the two paths of load/store stay ds_* and global_*
-// instructions: through a generic pointer the compiler merges them into one
-// flat access, which then waits on both the LDS and the memory counters.
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
+// LDS-typed (lds_u32), so that the two paths of load/store stay ds_* and
+// global_* instructions: through a generic pointer the compiler merges them
+// into one flat access, which then waits on both the LDS and the memory
+// counters.
 
 struct LowQueues {
   lds_u32* lds;  // [queue][vertex, row info, tag][cap]
@@ -272,8 +287,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   // one bit per vertex in LDS (lds_flags): SSSP's far flags (else a word per
   // vertex in HBM); BFS's visited marks, which then replace the atomic on
   // dist (else the atomic minimum decides the discovery)
-  lds_u32* fbits = (lds_u32*)(reinterpret_cast<uint32_t*>(lds) + (frontier_lds_bytes((int)blockDim.x) +
-                                                                   2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
+  lds_u32* fbits = batch_bits(lds);
   const bool bits = near_far && lds_flags;
   const bool vis = !SSSP && lds_flags;
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
@@ -585,10 +599,11 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       nf.cnt = b.fcnt + (size_t)j * 3 * kLine;
     }
     if (jb.team <= 1) {
+      // per-vertex bits in LDS when the bitmap holds the graph
+      const bool lds_flags = b.far_bits && batch_far_bits_bytes((int)blockDim.x) >= 4 * ((jb.n + 31) / 32);
       // degree-bounded step for graphs whose out-degree never exceeds 4
       // (row starts must fit row_info's 29 bits)
       if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
-        const bool lds_flags = b.far_bits && batch_far_bits_bytes((int)blockDim.x) >= 4 * ((jb.n + 31) / 32);
         if (b.low_ell && jb.e_off >= 0) {
           const uint32_t* ell = b.ell + jb.e_off;
           if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
@@ -599,8 +614,8 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
           else run_low<false, kLowDegree, kLowSlots, false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, nullptr);
         }
       } else {
-        if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
-        else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j]);
+        if (sssp) run_single<true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags);
+        else run_single<false>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags);
       }
     } else {
       int* tcnt = b.tbar + (size_t)j * 2 * kLine;

@@ -77,6 +77,11 @@ __host__ __device__ constexpr int heavy_threshold(int bd) { return 4 * bd > kHea
 // Dynamic LDS layout for one workgroup of bd threads: end[bd], beg[bd], du[bd], wave[17], claim[1]
 __host__ __device__ constexpr int frontier_lds_bytes(int bd) { return (3 * bd + 18) * 4; }
 
+// LDS-typed pointer: loads, stores and atomics through it stay ds_*
+// instructions (through a generic pointer they become flat accesses, which
+// wait on both the LDS and the memory counters).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 // An empty register barrier: the compiler may not fold a later use of v back
 // into the conditional block that produced it, where it would wait for the
 // load or atomic before the next slot's is issued.
@@ -88,17 +93,26 @@ __device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
 // per wave for the next level's queue (with distance tags under PAIRS) and
 // one for the far queue. Called by every lane of the wave; `active` marks
 // lanes with an edge.
+// bits (one workgroup owning the graph, LOCAL): one bit per vertex in LDS --
+// BFS marks a vertex visited there and the discoverer stores its level (no
+// read of dist and no atomic on it); SSSP keeps its far flags there instead
+// of in_far.
 template <bool SSSP, bool LOCAL, bool COH, bool PAIRS>
 __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev& s, bool active, int idx, uint32_t du,
                                            int32_t* out_q, uint32_t* out_qa, int* out_cnt, uint32_t next,
-                                           uint32_t thr, int32_t* far_q, int* far_cnt, uint32_t* in_far) {
+                                           uint32_t thr, int32_t* far_q, int* far_cnt, uint32_t* in_far,
+                                           lds_u32* bits = nullptr) {
   constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
   bool want = false, far_want = false;
   int u = 0;
   uint32_t alt = 0;
   if (active) {
     u = g.col[idx];
-    if (!SSSP) {
+    const uint32_t bit = 1u << (u & 31);
+    if (!SSSP && LOCAL && bits) {
+      want = (__hip_atomic_fetch_or(&bits[u >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit) == 0u;
+      if (want) s.dist[u] = next;
+    } else if (!SSSP) {
       if (s.dist[u] == kInf) {
         uint32_t expect = kInf;
         want = __hip_atomic_compare_exchange_strong(&s.dist[u], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -110,6 +124,9 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
         const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
         if (alt < old) {
           if (alt < thr) want = PAIRS || __hip_atomic_fetch_max(&s.stamp[u], next, __ATOMIC_RELAXED, scope) < next;
+          else if (LOCAL && bits)
+            far_want = (__hip_atomic_fetch_or(&bits[u >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                        bit) == 0u;
           else far_want = __hip_atomic_exchange(&in_far[u], 1u, __ATOMIC_RELAXED, scope) == 0u;
         }
       }
@@ -177,7 +194,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
                                                uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
                                                int* far_cnt = nullptr, uint32_t* in_far = nullptr,
                                                const uint32_t* in_qa = nullptr, uint32_t* out_qa = nullptr,
-                                               int* heavy_cnt = nullptr) {
+                                               int* heavy_cnt = nullptr, lds_u32* bits = nullptr) {
   const int bd = blockDim.x, tid = threadIdx.x;
   int* s_end = lds;
   int* s_beg = lds + bd;
@@ -240,7 +257,7 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
       du_e = SSSP ? s_du[lo] : 0u;
     }
     relax_push<SSSP, LOCAL, COH, PAIRS>(g, s, e < total, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q, far_cnt,
-                                        in_far);
+                                        in_far, bits);
   }
   __syncthreads();
 }
