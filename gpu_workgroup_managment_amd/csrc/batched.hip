@@ -82,6 +82,7 @@ struct BatchJob {
   int qcap;           // entries per queue: n + 1 (BFS), max(n, m) + 1 (SSSP, one entry per relaxation)
   uint32_t delta;     // SSSP near-far band width of this query (0: off)
   long long e_off;    // into the packed-row pool (kLowDegree words per vertex), -1: none
+  long long p_off;    // into the packed-edge pool (one word per edge, SSSP), -1: none
 };
 
 struct BatchDev {
@@ -111,6 +112,7 @@ struct BatchDev {
   unsigned long long* snap;     // control cells copied for the host: clock[0], clock[1], count, short team
   int far_bits;                 // 1: low-degree single-workgroup queries keep per-vertex bits in LDS (far / visited)
   const uint32_t* ell;          // packed rows of the low-degree graphs (BatchJob::e_off)
+  const uint32_t* pw;           // packed edges of the other SSSP graphs (BatchJob::p_off)
   int low_ell;                  // 1: low-degree single-workgroup queries read the packed rows
 };
 
@@ -576,6 +578,7 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
     g.wt = b.wt ? b.wt + jb.col_off : nullptr;
     g.n = jb.n;
     g.m = jb.m;
+    g.pw = jb.p_off >= 0 ? b.pw + jb.p_off : nullptr;
     FrontierDev s{};
     s.dist = b.dist + jb.v_off;
     s.stamp = nullptr;  // SSSP queues carry distance tags instead
@@ -625,6 +628,23 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       auto bar = [&]() { team_barrier(tcnt, tgen, team, epoch); };
       if (sssp) run_team<true>(g, s, nf, jb.src, r, jb.team, lds, &b.iters[j], bar);
       else run_team<false>(g, s, nf, jb.src, r, jb.team, lds, &b.iters[j], bar);
+    }
+    // The job's reached-edge sum (out-degrees of its reached vertices), by
+    // the workgroups that ran it, each over its share of the vertices. A
+    // team left its traversal after a barrier that drained every member's
+    // atomics, and reads dist coherently (its members' L2s may hold stale
+    // lines); one workgroup reads its own writes.
+    {
+      int lo = 0, hi = jb.n;
+      if (jb.team > 1) block_range(jb.n, r, jb.team, &lo, &hi);
+      unsigned long long local = 0;
+      for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+        const uint32_t d = jb.team > 1 ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                       : s.dist[v];
+        if (d != kInf) local += (unsigned long long)(g.row_ptr[v + 1] - g.row_ptr[v]);
+      }
+      for (int sh = 32; sh > 0; sh >>= 1) local += __shfl_down(local, sh, 64);
+      if ((threadIdx.x & 63) == 0 && local) atomicAdd(&b.reached[j], local);
     }
     // Thread 0 stamps the end before the loop's closing barrier: a thread-0
     // section after it would sit next to the ticket claim across the back
@@ -716,9 +736,10 @@ struct wgm_batch {
   wgm_runtime* rt = nullptr;
   int count = 0;
   std::vector<int32_t> n, query, src, maxdeg;
-  std::vector<int64_t> m, row_off, col_off, v_off, q_off, e_off;
-  int64_t total_rows = 0, total_edges = 0, total_v = 0, total_q = 0, total_e = 0;
+  std::vector<int64_t> m, row_off, col_off, v_off, q_off, e_off, p_off;
+  int64_t total_rows = 0, total_edges = 0, total_v = 0, total_q = 0, total_e = 0, total_p = 0;
   uint32_t* ell = nullptr;  // packed rows of the low-degree graphs (e_off, -1: none)
+  uint32_t* pw = nullptr;   // packed edges of the other SSSP graphs (p_off, -1: none)
   bool weighted = false;
   int32_t *row = nullptr, *col = nullptr, *wt = nullptr, *q = nullptr, *counts = nullptr, *tbar = nullptr;
   int32_t* claims = nullptr;
@@ -801,6 +822,7 @@ BatchDev dev_view(wgm_batch* b) {
   d.in_far = b->in_far;
   d.fcnt = b->fcnt;
   d.ell = b->ell;
+  d.pw = b->pw;
   return d;
 }
 
@@ -856,6 +878,7 @@ int build_jobs(wgm_batch* b, const wgm_batch_config& cfg, int grid) {
     j.qcap = b->qcap[i];
     j.delta = (uint32_t)(b->maxdeg[i] > kLowDegree && cfg.delta_high_degree > 0 ? cfg.delta_high_degree : cfg.delta);
     j.e_off = b->e_off[i];
+    j.p_off = b->p_off[i];
     ticket += j.team;
   }
   b->chunk = cfg.chunk;
@@ -880,11 +903,11 @@ int batch_grid(wgm_runtime* rt, const wgm_batch_config& cfg, int* grid) {
 // Enqueue one persistent batched launch (reset kernels + the kernel).
 // After the traversal: the reached-degree sums and the control snapshot
 // (part of the captured launch, so a run ends with one copy to the host).
-int enqueue_collect(wgm_batch* b, hipStream_t st) {
+int enqueue_collect(wgm_batch* b, hipStream_t st, bool reached) {
   wgm_runtime* rt = b->rt;
   const BatchDev d = dev_view(b);
   const int max_n = *std::max_element(b->n.begin(), b->n.end());
-  for (int j0 = 0; j0 < b->count; j0 += kMaxGridY)
+  for (int j0 = 0; reached && j0 < b->count; j0 += kMaxGridY)
     hipLaunchKernelGGL(k_batch_reached, dim3((max_n + kReachChunk - 1) / kReachChunk, std::min(kMaxGridY, b->count - j0)),
                        dim3(256), 0, st, d, j0);
   hipLaunchKernelGGL(k_batch_snap, dim3(1), dim3(64), 0, st, rt->ctl, d);
@@ -901,7 +924,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   d.far_bits = cfg.far_flags_lds ? 1 : 0;
   d.low_ell = cfg.low_ell ? 1 : 0;
   hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
-  return enqueue_collect(b, st);
+  return enqueue_collect(b, st, false);  // k_batched sums each job's reached edges itself
 }
 
 // Ends a run: waits for the stream (event ev1 after the work), copies the
@@ -1013,10 +1036,14 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     b->v_off.push_back(b->total_v);
     b->q_off.push_back(b->total_q);
     b->f_off.push_back(b->total_f);
-    bool packed = md <= kLowDegree && g.n <= kEllMaxN;
-    for (int64_t k = 0; packed && g.wt && k < g.m; k++) packed = g.wt[k] >= 0 && g.wt[k] <= 255;
-    b->e_off.push_back(packed ? b->total_e : -1);
-    if (packed) b->total_e += ((int64_t)kLowDegree * g.n + 63) / 64 * 64;
+    bool w8 = g.n <= kEllMaxN;  // vertex ids fit 24 bits and weights 8
+    for (int64_t k = 0; w8 && g.wt && k < g.m; k++) w8 = g.wt[k] >= 0 && g.wt[k] <= 255;
+    const bool rows = w8 && md <= kLowDegree;
+    const bool edges = w8 && md > kLowDegree && queries[i] == WGM_QUERY_SSSP;
+    b->e_off.push_back(rows ? b->total_e : -1);
+    if (rows) b->total_e += ((int64_t)kLowDegree * g.n + 63) / 64 * 64;
+    b->p_off.push_back(edges ? b->total_p : -1);
+    if (edges) b->total_p += ((int64_t)g.m + 63) / 64 * 64;
     // SSSP queues carry distance tags and a level queues at most one entry
     // per relaxation; BFS claims each vertex once
     const int64_t cap = (queries[i] == WGM_QUERY_SSSP ? std::max<int64_t>(g.n, g.m) : g.n) + 1;
@@ -1054,6 +1081,7 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
   if (e == hipSuccess) e = alloc((void**)&b->in_far, sizeof(uint32_t) * b->total_v);
   if (e == hipSuccess) e = alloc((void**)&b->fcnt, sizeof(int32_t) * 3 * kLine * count);
   if (e == hipSuccess) e = alloc((void**)&b->ell, sizeof(uint32_t) * b->total_e);
+  if (e == hipSuccess) e = alloc((void**)&b->pw, sizeof(uint32_t) * b->total_p);
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "hipMalloc(batch pools)"));
   // zeroed state: a queue slot read before it is written holds a vertex id of
   // its own job (0 at first), never garbage
@@ -1092,6 +1120,16 @@ int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, cons
     }
     e = hipMemcpy(b->ell, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice);
   }
+  if (b->total_p > 0 && e == hipSuccess) {
+    std::vector<uint32_t> words((size_t)b->total_p, 0u);
+    for (int i = 0; i < count; i++) {
+      if (b->p_off[i] < 0) continue;
+      const wgm_csr& g = graphs[i];
+      uint32_t* w = words.data() + b->p_off[i];
+      for (int64_t k = 0; k < g.m; k++) w[k] = ((uint32_t)g.col[k] << 8) | (uint32_t)g.wt[k];
+    }
+    e = hipMemcpy(b->pw, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) return bail(wgm_hip_fail(rt, e, "batch upload"));
   *out = b;
   return WGM_OK;
@@ -1102,7 +1140,7 @@ int wgm_batch_destroy(wgm_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void* ptrs[] = {b->row, b->col, b->wt, b->q, b->counts, b->tbar, b->ticket, b->res, b->dist, b->qa, b->qr,
-                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->sa_heavy, b->ell};
+                  b->d_jobs, b->claims, b->d_jobs_one, b->far, b->in_far, b->fcnt, b->sa_heavy, b->ell, b->pw};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete b;
@@ -1111,7 +1149,7 @@ int wgm_batch_destroy(wgm_batch* b) {
 
 int64_t wgm_batch_bytes(const wgm_batch* b) {
   if (!b) return 0;
-  return 4 * (b->total_rows + b->total_edges * (b->weighted ? 2 : 1) + b->total_e);
+  return 4 * (b->total_rows + b->total_edges * (b->weighted ? 2 : 1) + b->total_e + b->total_p);
 }
 
 int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg_in, wgm_batch_stats* st) {
@@ -1209,7 +1247,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
                        rt->stream, rt->ctl, one);
   }
   HIPCHK(rt, hipGetLastError());
-  rc = enqueue_collect(b, rt->stream);
+  rc = enqueue_collect(b, rt->stream, false);
   if (rc) return rc;
   wgm_batch_stats local{};
   local.launches = b->count;
@@ -1290,7 +1328,7 @@ int wgm_batch_run_standalone(wgm_runtime* rt, wgm_batch* b, const wgm_launch* lc
     if (!rc) rc = wgm_enqueue_frontier(rt, g, s, nf, sssp, j.src, sssp ? grid_sssp : grid_bfs, wg, d.iters + k);
     if (rc) return rc;
   }
-  rc = enqueue_collect(b, rt->stream);
+  rc = enqueue_collect(b, rt->stream, true);
   if (rc) return rc;
   wgm_batch_stats local{};
   local.launches = b->count;

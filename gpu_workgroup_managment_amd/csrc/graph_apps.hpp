@@ -25,6 +25,9 @@ struct GraphDev {
   const uint32_t* prio;
   int32_t n;
   int32_t m;
+  // SSSP: one word per edge, (col << 8) | wt, for graphs whose weights fit 8
+  // bits (nullptr: col and wt); one load per relaxed edge instead of two
+  const uint32_t* pw;
 };
 
 // Frontier state. Level L's frontier is q[L & 1] with its length in
@@ -107,7 +110,15 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
   int u = 0;
   uint32_t alt = 0;
   if (active) {
-    u = g.col[idx];
+    uint32_t w = 0u;
+    if (SSSP && g.pw) {
+      const uint32_t word = g.pw[idx];
+      u = (int)(word >> 8);
+      w = word & 255u;
+    } else {
+      u = g.col[idx];
+      if (SSSP) w = (uint32_t)g.wt[idx];
+    }
     const uint32_t bit = 1u << (u & 31);
     if (!SSSP && LOCAL && bits) {
       want = (__hip_atomic_fetch_or(&bits[u >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit) == 0u;
@@ -119,7 +130,7 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
                                                     scope);
       }
     } else {
-      alt = du + (uint32_t)g.wt[idx];
+      alt = du + w;
       if (alt < s.dist[u]) {
         const uint32_t old = __hip_atomic_fetch_min(&s.dist[u], alt, __ATOMIC_RELAXED, scope);
         if (alt < old) {

@@ -724,7 +724,7 @@ typedef struct wgm_batch_stats {
 int wgm_batch_create(wgm_runtime* rt, int32_t count, const wgm_csr* graphs, const int32_t* queries,
                      const int32_t* sources, wgm_batch** out);
 int wgm_batch_destroy(wgm_batch* b);
-int64_t wgm_batch_bytes(const wgm_batch* b); /* resident graph bytes: CSR, weights and packed rows */
+int64_t wgm_batch_bytes(const wgm_batch* b); /* resident graph bytes: CSR, weights, packed rows and edges */
 /* One persistent launch serving every query (cfg may be NULL for defaults). */
 int wgm_batch_run(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config* cfg, wgm_batch_stats* st);
 /* Baseline: one persistent launch per graph, back to back. */

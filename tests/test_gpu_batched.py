@@ -194,3 +194,26 @@ def test_batched_packed_rows_skip_wide_weights(rt):
     b.run(wg_size=1024, wgs_per_cu=1, delta=64, far_flags_lds=1, low_ell=1)
     _check_all(b, graphs, refs)
     b.close()
+
+
+@pytest.mark.parametrize("team_edges", [1 << 30, 2048])
+def test_batched_packed_edges_skip_wide_weights(rt, team_edges):
+    """SSSP on higher-degree graphs reads one packed word per edge ((target <<
+    8) | weight) on one workgroup and in teams; a graph with a weight above 255
+    keeps col and wt; BFS graphs get no packed edges. Every query exact."""
+    from gpu_workgroup_managment_amd import graphs as G
+    wide = G.add_weights(G.rmat(10, 8, seed=5), seed=7)
+    wide.wt[3] = 4096
+    graphs = [G.add_weights(G.rmat(11, 8, seed=2), seed=1), wide, G.rmat(10, 8, seed=6)]
+    queries = np.array([L.QUERY_SSSP, L.QUERY_SSSP, L.QUERY_BFS], np.int32)
+    refs = [oracle.sssp(graphs[0].row_ptr, graphs[0].col, graphs[0].wt, 0),
+            oracle.sssp(wide.row_ptr, wide.col, wide.wt, 0), oracle.bfs(graphs[2].row_ptr, graphs[2].col, 0)[0]]
+    assert all(g.degrees().max() > 4 for g in graphs)
+    b = B.Batch(rt, graphs, queries)
+    packed = (graphs[0].m + 63) // 64 * 64
+    assert b.nbytes() == 4 * (sum(g.n + 1 + 2 * g.m for g in graphs) + packed)
+    for flags in (1, 0):
+        b.run(wg_size=1024, wgs_per_cu=1, team_edges=team_edges, max_team=8, delta=64, delta_high_degree=8,
+              far_flags_lds=flags, low_ell=1)
+        _check_all(b, graphs, refs)
+    b.close()
