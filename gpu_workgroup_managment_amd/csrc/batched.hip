@@ -59,6 +59,8 @@ __host__ __device__ constexpr int batch_far_bits_bytes(int bd) { return bd >= 10
 // | weight, kEllNone past the degree. A graph gets them when every weight fits
 // 8 bits and its vertex ids fit 24 (kEllNone is no valid word then).
 constexpr uint32_t kEllNone = 0xffffffffu;
+// vertices per thread per round of a reached-edge sum (loads in flight)
+constexpr int kReachPer = 8;
 constexpr int32_t kEllMaxN = (1 << 24) - 1;
 __host__ __device__ constexpr int batch_lds_bytes(int bd) {
   return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd);
@@ -424,6 +426,19 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
 #pragma unroll
         for (int x = 0; x < U; x++) {
           if (!lv[x]) continue;
+          // the far flag's exchange is issued first, so that its round trip
+          // overlaps the near queue's counter atomic below
+          bool far_want = false;
+          if (SSSP) {
+            if (bits) {
+              const uint32_t bit = 1u << (u[x] & 31);
+              far_want = far_cand[x] && (__hip_atomic_fetch_or(&fbits[u[x] >> 5], bit, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP) & bit) == 0u;
+            } else {
+              far_want = far_cand[x] && __hip_atomic_exchange(&nf.flag[u[x]], 1u, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+            }
+          }
           const unsigned long long mask = __ballot(want[x]);
           if (mask) {
             const int leader = __ffsll((long long)mask) - 1;
@@ -433,18 +448,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
             if (want[x]) lq.store(1 - p, base + __popcll(mask & below), u[x], ru[x], alt[x], SSSP, !ELL);
           }
-          if (SSSP) {
-            bool far_want;
-            if (bits) {
-              const uint32_t bit = 1u << (u[x] & 31);
-              far_want = far_cand[x] && (__hip_atomic_fetch_or(&fbits[u[x] >> 5], bit, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_WORKGROUP) & bit) == 0u;
-            } else {
-              far_want = far_cand[x] && __hip_atomic_exchange(&nf.flag[u[x]], 1u, __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
-            }
-            push_item<false>(far_want, u[x], nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
+          if (SSSP) push_item<false>(far_want, u[x], nf.far(fi), &s_far[fi], __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
       __syncthreads();
@@ -638,10 +642,25 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       int lo = 0, hi = jb.n;
       if (jb.team > 1) block_range(jb.n, r, jb.team, &lo, &hi);
       unsigned long long local = 0;
-      for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-        const uint32_t d = jb.team > 1 ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                       : s.dist[v];
-        if (d != kInf) local += (unsigned long long)(g.row_ptr[v + 1] - g.row_ptr[v]);
+      // kReachPer vertices per thread per round, all loads issued before any
+      // use: one memory round trip per kReachPer * blockDim.x vertices (the
+      // largest grid's workgroup sweeps 257k vertices at the end of the step)
+      for (int base = lo; base < hi; base += kReachPer * (int)blockDim.x) {
+        uint32_t d[kReachPer];
+        int r0[kReachPer], r1[kReachPer];
+#pragma unroll
+        for (int q = 0; q < kReachPer; q++) {
+          const int v = base + q * (int)blockDim.x + (int)threadIdx.x;
+          const bool in = v < hi;
+          d[q] = kInf;
+          if (in) d[q] = jb.team > 1 ? __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : s.dist[v];
+          r0[q] = in ? g.row_ptr[v] : 0;
+          r1[q] = in ? g.row_ptr[v + 1] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kReachPer; q++)
+          if (d[q] != kInf) local += (unsigned long long)(r1[q] - r0[q]);
       }
       for (int sh = 32; sh > 0; sh >>= 1) local += __shfl_down(local, sh, 64);
       if ((threadIdx.x & 63) == 0 && local) atomicAdd(&b.reached[j], local);
@@ -682,7 +701,6 @@ __global__ void k_batch_reset(BatchDev b) {
 // vertices (the blocks past a small job's end leave at once). Each thread
 // issues all kReachPer of its loads before summing (one round of memory
 // latency per block), then one atomic per block.
-constexpr int kReachPer = 8;
 constexpr int kReachChunk = 256 * kReachPer;
 constexpr int kMaxGridY = 65535;
 __global__ void __launch_bounds__(256) k_batch_reached(BatchDev b, int job0) {
