@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(1024) k_color(Ctl c, GraphDev g, int32_t* colo
   int* flags = c.lines;
   int round = 1;
   for (;; round++) {
-    const bool any = color_round(g, color, round, lo, hi);
+    const bool any = color_round<true>(g, color, round, lo, hi);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
     if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
     xcd_barrier(c, t);
@@ -74,12 +74,12 @@ __global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, in
   int* flags = c.lines;
   int round = 1;
   for (;; round++) {
-    const bool any = mis_phase_a(g, st, s_out, round, lo, hi);
+    const bool any = mis_phase_a<true>(g, st, s_out, round, lo, hi);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
     if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
     xcd_barrier(c, t);
     if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
-    mis_phase_b(g, st, round, lo, hi);
+    mis_phase_b<true>(g, st, round, lo, hi);
     xcd_barrier(c, t);
   }
   clock_end(c, t.id);

@@ -91,6 +91,19 @@ def test_color_er_with_degree_one(rt):
     assert np.array_equal(col, ref), _first_diff(col, ref)
 
 
+def test_color_rmat(rt):
+    """Vertices of high degree: the edges past the first four of a vertex
+    are walked after the grouped loads."""
+    g = G.rmat(12, 8, seed=5)
+    src = np.repeat(np.arange(g.n), g.degrees())
+    g = G.add_priorities(G.from_edges(g.n, src, g.col, symmetric=True))
+    assert g.degrees().max() > 64
+    for wg, wgs in ((256, 32), (1024, 0)):
+        col, _ = rt.graph(g).color(wg_size=wg, num_wgs=wgs)
+        ref, _ = oracle.color(g.row_ptr, g.col, g.prio)
+        assert np.array_equal(col, ref), _first_diff(col, ref)
+
+
 @pytest.mark.parametrize("side,wgs", [(64, 4), (512, 0)])
 def test_mis_grid(rt, side, wgs):
     g = G.add_priorities(G.grid(side, side))
