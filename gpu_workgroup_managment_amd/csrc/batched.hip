@@ -1386,86 +1386,92 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
                        wgm_batch_stats* best_stats) {
   if (!rt || !b || !best_out) return fail(rt, WGM_ERR_INVALID, "bad autotune arguments");
   if (reps < 1) reps = 1;
-  const int wgs[] = {64, 128, 256, 512, 1024};
-  const int per_cu[] = {1, 2, 4, 8};
-  const int team_edges[] = {1 << 14, 1 << 16, 1 << 18, 1 << 20};
-  const int chunks[] = {32, 256, 4096};
   wgm_batch_config best{};
   wgm_batch_stats bst{};
   double best_ms = 1e30;
+  auto trial = [&](const wgm_batch_config& c) {
+    for (int r = 0; r < reps; r++) {
+      wgm_batch_stats s{};
+      const int rc = wgm_batch_run(rt, b, &c, &s);
+      if (rc) return rc;
+      if (s.wall_ms < best_ms) {
+        best_ms = s.wall_ms;
+        best = c;
+        bst = s;
+      }
+    }
+    return (int)WGM_OK;
+  };
+  // the other knobs start at values that suit a mixed batch: BFS teams of
+  // 2^20 edges per workgroup, SSSP teams of 786432, bands of 512 (low
+  // degree) and 12, the LDS bits and packed rows on
+  wgm_batch_config start{};
+  start.chunk = 256;
+  start.team_edges = 1 << 20;
+  start.max_team = 16;
+  start.delta = 512;
+  start.delta_high_degree = 12;
+  start.sssp_team_edges = 786432;
+  start.far_flags_lds = 1;
+  start.low_ell = 1;
+  // 1. launch shape: workgroup size x workgroups per CU
+  const int wgs[] = {64, 128, 256, 512, 1024};
+  const int per_cu[] = {1, 2, 4, 8};
   for (int wg : wgs)
     for (int pc : per_cu) {
       if (wg * pc > 2048) continue;  // CU thread capacity
-      for (int te : team_edges)
-        for (int ch : chunks) {
-          if (ch != 256 && te != (1 << 16)) continue;  // chunk only swept at the middle granularity
-          wgm_batch_config c{};
-          c.wg_size = wg;
-          c.wgs_per_cu = pc;
-          c.team_edges = te;
-          c.chunk = ch;
-          c.max_team = 16;
-          c.use_graph = 0;
-          c.delta = 1024;
-          for (int r = 0; r < reps; r++) {
-            wgm_batch_stats s{};
-            int rc = wgm_batch_run(rt, b, &c, &s);
-            if (rc) return rc;
-            if (s.wall_ms < best_ms) {
-              best_ms = s.wall_ms;
-              best = c;
-              bst = s;
-            }
-          }
-        }
-    }
-  // then the SSSP near-far band at the best launch shape: the band of
-  // low-degree graphs, then that of the others
-  const int deltas[] = {0, 512, 2048, 4096};
-  for (int dl : deltas) {
-    wgm_batch_config c = best;
-    c.delta = dl;
-    for (int r = 0; r < reps; r++) {
-      wgm_batch_stats s{};
-      int rc = wgm_batch_run(rt, b, &c, &s);
+      wgm_batch_config c = start;
+      c.wg_size = wg;
+      c.wgs_per_cu = pc;
+      const int rc = trial(c);
       if (rc) return rc;
-      if (s.wall_ms < best_ms) {
-        best_ms = s.wall_ms;
-        best = c;
-        bst = s;
-      }
+    }
+  // 2. then one knob at a time at the best so far, twice over the knobs
+  //    that interact (team granularities and bands)
+  const int chunks[] = {32, 64, 256, 1024, 4096};
+  for (int ch : chunks) {
+    wgm_batch_config c = best;
+    c.chunk = ch;
+    const int rc = trial(c);
+    if (rc) return rc;
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    const int team_edges[] = {1 << 14, 1 << 16, 1 << 18, 1 << 20, 1 << 22};
+    for (int te : team_edges) {
+      wgm_batch_config c = best;
+      c.team_edges = te;
+      const int rc = trial(c);
+      if (rc) return rc;
+    }
+    const int sssp_te[] = {0, 1 << 16, 1 << 18, 1 << 19, 786432, 1 << 20};
+    for (int te : sssp_te) {
+      wgm_batch_config c = best;
+      c.sssp_team_edges = te;
+      const int rc = trial(c);
+      if (rc) return rc;
+    }
+    const int deltas[] = {0, 256, 512, 1024, 2048, 4096};
+    for (int dl : deltas) {
+      wgm_batch_config c = best;
+      c.delta = dl;
+      const int rc = trial(c);
+      if (rc) return rc;
+    }
+    const int deltas_hi[] = {0, 4, 8, 12, 16, 32, 128};
+    for (int dl : deltas_hi) {
+      wgm_batch_config c = best;
+      c.delta_high_degree = dl;
+      const int rc = trial(c);
+      if (rc) return rc;
     }
   }
-  const int deltas_hi[] = {4, 8, 32, 128};
-  for (int dl : deltas_hi) {
+  // 3. the single-workgroup step's per-vertex LDS bits and packed rows
+  for (int flags = 0; flags < 4; flags++) {
     wgm_batch_config c = best;
-    c.delta_high_degree = dl;
-    for (int r = 0; r < reps; r++) {
-      wgm_batch_stats s{};
-      int rc = wgm_batch_run(rt, b, &c, &s);
-      if (rc) return rc;
-      if (s.wall_ms < best_ms) {
-        best_ms = s.wall_ms;
-        best = c;
-        bst = s;
-      }
-    }
-  }
-  // and the SSSP team granularity at the best shape and band
-  const int sssp_te[] = {1 << 16, 1 << 17, 1 << 18, 1 << 19};
-  for (int te : sssp_te) {
-    wgm_batch_config c = best;
-    c.sssp_team_edges = te;
-    for (int r = 0; r < reps; r++) {
-      wgm_batch_stats s{};
-      int rc = wgm_batch_run(rt, b, &c, &s);
-      if (rc) return rc;
-      if (s.wall_ms < best_ms) {
-        best_ms = s.wall_ms;
-        best = c;
-        bst = s;
-      }
-    }
+    c.far_flags_lds = flags & 1;
+    c.low_ell = flags >> 1;
+    const int rc = trial(c);
+    if (rc) return rc;
   }
   *best_out = best;
   if (best_stats) *best_stats = bst;

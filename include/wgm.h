@@ -739,11 +739,13 @@ int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* i
  * microseconds of the device wall clock from the first query's start; each
  * array holds `count` values in caller order (-1: the query did not run). */
 int wgm_batch_job_times(const wgm_batch* b, double* start_us, double* end_us);
-/* Grid search over wg_size {64..1024}, wgs_per_cu {1,2,4,8}, team_edges
- * {2^14..2^20} and chunk {32..4096} (band 1024), then the SSSP near-far band
- * {0, 512..4096}, the band of graphs with an out-degree above 4 {4..128} and
- * the SSSP team granularity {2^16..2^19} at the best shape; returns the
- * fastest configuration. */
+/* Search: wg_size {64..1024} x wgs_per_cu {1,2,4,8} from a start suited to
+ * mixed batches (team_edges 2^20, sssp_team_edges 786432, bands 512 / 12,
+ * LDS bits and packed rows on), then one knob at a time at the best so far:
+ * chunk {32..4096}; twice over team_edges {2^14..2^22}, sssp_team_edges
+ * {0, 2^16..2^20}, the SSSP band {0, 256..4096} and the band of graphs with an
+ * out-degree above 4 {0, 4..128}; then far_flags_lds x low_ell {0,1}^2.
+ * Each candidate runs `reps` times; returns the fastest configuration. */
 int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best,
                        wgm_batch_stats* best_stats);
 

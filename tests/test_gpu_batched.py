@@ -73,8 +73,13 @@ def test_batched_autotune_returns_an_exact_config(rt, small_batch):
     cfg = best.as_dict()
     print(cfg, st["wall_ms"])
     assert cfg["wg_size"] in (64, 128, 256, 512, 1024)
-    assert cfg["delta"] in (0, 512, 1024, 2048, 4096)
-    assert cfg["delta_high_degree"] in (0, 4, 8, 32, 128)
+    assert cfg["wg_size"] * cfg["wgs_per_cu"] <= 2048
+    assert cfg["chunk"] in (32, 64, 256, 1024, 4096)
+    assert cfg["team_edges"] in (1 << 14, 1 << 16, 1 << 18, 1 << 20, 1 << 22)
+    assert cfg["delta"] in (0, 256, 512, 1024, 2048, 4096)
+    assert cfg["delta_high_degree"] in (0, 4, 8, 12, 16, 32, 128)
+    assert cfg["sssp_team_edges"] in (0, 1 << 16, 1 << 18, 1 << 19, 786432, 1 << 20)
+    assert cfg["far_flags_lds"] in (0, 1) and cfg["low_ell"] in (0, 1)
     assert st["wall_ms"] > 0
     b.run(best)
     _check_all(b, graphs, refs)
