@@ -61,9 +61,25 @@ __global__ void __launch_bounds__(1024) k_minplus(Ctl c, MinplusDev p, int* iter
     for (int i = first; i < p.ntasks; i += stride) {
       const int4 tk = p.tasks[i];
       int best = INT32_MAX;
-      for (int e = tk.y + lane; e < tk.z; e += G) {
-        // a plain (workgroup-scope) read: a value another workgroup lowers
-        // during this sweep may be seen late; the barrier makes it current
+      // a plain (workgroup-scope) read of dist: a value another workgroup
+      // lowers during this sweep may be seen late; the barrier makes it
+      // current. Four of the lane's edges per round: their sources and
+      // weights, then the four gathers, so the lane waits for two memory
+      // round trips per four edges rather than per edge.
+      int e = tk.y + lane;
+      for (; e + 3 * G < tk.z; e += 4 * G) {
+        int sv[4], wv[4], dv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          sv[q] = p.rcol[e + q * G];
+          wv[q] = p.rwt[e + q * G];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) dv[q] = __hip_atomic_load(&p.dist[sv[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int q = 0; q < 4; q++) best = wv[q] + dv[q] < best ? wv[q] + dv[q] : best;
+      }
+      for (; e < tk.z; e += G) {
         const int du = __hip_atomic_load(&p.dist[p.rcol[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const int cand = p.rwt[e] + du;
         best = cand < best ? cand : best;
