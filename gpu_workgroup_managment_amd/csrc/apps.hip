@@ -453,9 +453,16 @@ static int run_frontier(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_la
   int grid = 0;
   rc = wgm_resolve_grid(rt, kernel, wg, lc ? lc->num_wgs : 0, &grid);
   if (rc) return rc;
+  // BFS takes its wide levels bottom-up over the in-edges
+  GraphDev view = wgm_graph_view(g);
+  if (!sssp) {
+    rc = wgm_graph_in_edges(g);
+    if (rc) return rc;
+    view.rrow = g->rrow;
+    view.rcol = g->rcol;
+  }
   wgm_run_stats local{};
-  rc = wgm_launch_frontier(rt, wgm_graph_view(g), s, nf, sssp, src, grid, wg, lc ? lc->handshake : 0, g->iters,
-                           &local);
+  rc = wgm_launch_frontier(rt, view, s, nf, sssp, src, grid, wg, lc ? lc->handshake : 0, g->iters, &local);
   if (rc) return rc;
   rc = wgm_graph_reached_edges(g, &local.edges_scanned);
   if (rc) return rc;

@@ -70,4 +70,6 @@ int wgm_enqueue_frontier(wgm_runtime* rt, const wgm::GraphDev& g, wgm::FrontierD
                          int src, int grid, int wg, int* iters_dev);
 uint32_t wgm_standalone_delta(int maxdeg);
 int wgm_sssp_pull(wgm_runtime* rt, wgm_graph* g, int32_t src, const wgm_launch* lc, wgm_run_stats* st);
+// the graph's in-edge CSR (g->rrow, g->rcol), built once and kept resident
+int wgm_graph_in_edges(wgm_graph* g);
 int wgm_enqueue_reached_edges(wgm_runtime* rt, const wgm::GraphDev& g, const uint32_t* dist, unsigned long long* out);

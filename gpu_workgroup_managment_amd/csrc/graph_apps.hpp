@@ -28,6 +28,10 @@ struct GraphDev {
   // SSSP: one word per edge, (col << 8) | wt, for graphs whose weights fit 8
   // bits (nullptr: col and wt); one load per relaxed edge instead of two
   const uint32_t* pw;
+  // in-edge CSR (sources of each vertex's in-edges); a team's BFS may then
+  // take its wide levels bottom-up (nullptr: top-down only)
+  const int32_t* rrow;
+  const int32_t* rcol;
 };
 
 // Frontier state. Level L's frontier is q[L & 1] with its length in

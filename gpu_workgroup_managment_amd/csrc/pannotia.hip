@@ -350,6 +350,8 @@ int bc_grid(wgm_runtime* rt, const wgm_graph* g, const wgm_launch* lc, int* grid
 
 }  // namespace
 
+int wgm_graph_in_edges(wgm_graph* g) { return graph_reverse(g); }
+
 bool wgm_pannotia_kernel_info(int kernel, KernelInfo* out) {
   switch (kernel) {
     case WGM_K_MINPLUS: *out = KernelInfo{(const void*)&k_minplus<16>, nullptr}; return true;

@@ -63,6 +63,60 @@ __device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epo
   __syncthreads();
 }
 
+// A BFS level taken bottom-up (direction-optimizing BFS, Beamer et al.,
+// SC 2012): every unvisited vertex of the member's range looks for an
+// in-neighbour in the frontier (dist == level), four in-edges per round with
+// an early exit, and if it finds one takes level + 1 and queues itself. A
+// vertex is written only by its own thread, so no atomic on dist is needed;
+// the distances are read and written with agent-scope accesses (coherent
+// across the XCDs' L2s, as the top-down step's). The set of vertices it
+// discovers is the top-down level's: the unvisited vertices with an in-edge
+// from the frontier. The level's counters are rotated as frontier_level
+// rotates them.
+constexpr int kBottomUpDiv = 16;  // bottom-up while the frontier holds more than n / 16 vertices
+
+__device__ __forceinline__ void frontier_bottom_up(const GraphDev& g, const FrontierDev& s, int level, int rank,
+                                                   int nteam) {
+  const int bd = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+  int32_t* out_q = (level & 1) ? s.q0 : s.q1;
+  int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
+  const uint32_t cur = (uint32_t)level, next = (uint32_t)level + 1u;
+  int lo, hi;
+  block_range(g.n, rank, nteam, &lo, &hi);
+  for (int base = lo; base < hi; base += bd) {
+    const int v = base + tid;
+    bool found = false;
+    if (v < hi && __hip_atomic_load(&s.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == kInf) {
+      const int b = g.rrow[v], e = g.rrow[v + 1];
+      for (int j = b; j < e && !found; j += 4) {
+        int u[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) u[q] = j + q < e ? g.rcol[j + q] : -1;
+        uint32_t du[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          du[q] = u[q] >= 0 ? __hip_atomic_load(&s.dist[u[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInf;
+#pragma unroll
+        for (int q = 0; q < 4; q++) found = found || du[q] == cur;
+      }
+      if (found) __hip_atomic_store(&s.dist[v], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const unsigned long long mask = __ballot(found);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      int pos = 0;
+      if (lane == leader) pos = __hip_atomic_fetch_add(out_cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      pos = __builtin_amdgcn_readlane(pos, leader) + __popcll(mask & ((1ull << lane) - 1ull));
+      if (found) __hip_atomic_store(&out_q[pos], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (rank == 0 && tid == 0) {
+    st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);
+    if (s.chunk > 0 && s.claims) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
+    if (s.heavy > 0) st_relaxed(&s.hcnt[((level + 1) % 3) * kLine], 0);
+  }
+}
+
 // A graph served by a team of k workgroups (rank 0..k-1). Everything the
 // team shares -- queue entries and their distance tags, a frontier vertex's
 // distance, counters -- goes through agent-scope atomics, so `bar` (the
@@ -103,9 +157,14 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
   int level = 0, fi = 0;
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   while (true) {
-    while (ld_relaxed(&s.counts[(level % 3) * kLine]) != 0) {
-      frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
-                                 nf.flag);
+    int in_n;
+    while ((in_n = ld_relaxed(&s.counts[(level % 3) * kLine])) != 0) {
+      // every member read the same count after the barrier: the same choice
+      if (!SSSP && g.rrow && in_n > g.n / kBottomUpDiv)
+        frontier_bottom_up(g, s, level, rank, k);
+      else
+        frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr,
+                                         &nf.cnt[fi * kLine], nf.flag);
       bar();
       // heavy entries deferred by the pass (read after the barrier: every
       // member sees the same count and takes the same number of barriers)

@@ -26,6 +26,21 @@ def test_bfs_er(rt, n, pairs, wgs):
     print(st)
 
 
+@pytest.mark.parametrize("wg,wgs", [(256, 16), (1024, 0)])
+def test_bfs_directed_rmat_bottom_up(rt, wg, wgs):
+    """A directed RMAT graph: its wide levels run bottom-up over the in-edges
+    (which differ from the out-edges here), the narrow ones top-down; levels,
+    level count and scanned edges are the oracle's."""
+    g = G.rmat(16, 16, seed=3)
+    src = int(np.argmax(g.degrees()))  # a hub: the second level is most of the graph
+    ref, levels = oracle.bfs(g.row_ptr, g.col, src)
+    assert int(np.bincount(ref[ref != oracle.INF]).max()) > g.n // 16  # some level is wide
+    dist, st = rt.graph(g).bfs(src, wg_size=wg, num_wgs=wgs)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["iterations"] == levels
+    assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())
+
+
 def test_bfs_grid_closed_form(rt):
     side = 300
     g = G.grid(side, side)
