@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # packed rows (7.00 vs 7.77 ms). The SSSP teams' granularity and band took the
 # step from 6.79 to about 6.1 ms.
 TUNED = dict(wg_size=1024, wgs_per_cu=2, chunk=256, team_edges=1 << 20, max_team=16, delta=512,
-             delta_high_degree=12, far_flags_lds=1, low_ell=1, sssp_team_edges=786432)
+             delta_high_degree=12, far_flags_lds=1, low_ell=2, sssp_team_edges=786432)
 
 
 def log(msg):
@@ -54,7 +54,7 @@ def parse():
                     help="apps: one line per standalone kernel on its workload (not the headline line)")
     ap.add_argument("--far-flags-lds", type=int, default=None, choices=(0, 1),
                     help="override the tuned setting of the LDS far-flag bitmap")
-    ap.add_argument("--low-ell", type=int, default=None, choices=(0, 1),
+    ap.add_argument("--low-ell", type=int, default=None, choices=(0, 1, 2),
                     help="override the tuned setting of the packed rows of low-degree graphs")
     ap.add_argument("--only", choices=("rmat-bfs", "rmat-sssp", "grid-bfs", "grid-sssp"),
                     help="time one class of the batched workload (not the headline line)")

@@ -16,7 +16,9 @@ def config(wg_size=256, wgs_per_cu=4, chunk=0, team_edges=1 << 18, max_team=16, 
     c.team_edges, c.max_team, c.use_graph = team_edges, max_team, 1 if use_graph else 0
     c.delta, c.sssp_team_edges, c.delta_high_degree = delta, sssp_team_edges, delta_high_degree
     c.far_flags_lds = 1 if far_flags_lds else 0
-    c.low_ell = 1 if low_ell else 0
+    if low_ell not in (0, 1, 2, False, True):
+        raise ValueError("low_ell is 0, 1 or 2")
+    c.low_ell = int(low_ell)
     return c
 
 

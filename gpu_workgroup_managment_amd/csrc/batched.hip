@@ -62,14 +62,64 @@ constexpr uint32_t kEllNone = 0xffffffffu;
 // vertices per thread per round of a reached-edge sum (loads in flight)
 constexpr int kReachPer = 8;
 constexpr int32_t kEllMaxN = (1 << 24) - 1;
+// Distances kept in the L2 (low_ell == 2): a low-degree SSSP query on one
+// 1024-thread workgroup updates its distances with plain loads and stores,
+// which stay in its XCD's L2, instead of atomics, which are all performed
+// beyond it. Each pass first folds its candidates per target in an LDS hash
+// table of kCombSlots 64-bit words, (distance << 32) | (target << 10) |
+// candidate (the thread, so that words are unique and the minimum has one
+// owner), open addressing with linear probes. A pass has at most blockDim.x
+// <= 1024 candidates, so a table is at most half full and every probe
+// sequence ends.
+// Three tables serve consecutive passes in turn: the winner of a slot clears
+// it after the next pass's barrier, when every reader of the slot is past it,
+// and the pass after that (the slot's next user) starts after the clear.
+constexpr int kCombBits = 11;
+constexpr int kCombSlots = 1 << kCombBits;
+constexpr unsigned long long kCombEmpty = ~0ull;
+__host__ __device__ constexpr int batch_comb_bytes(int bd) { return bd >= 1024 ? 3 * kCombSlots * 8 : 0; }
 __host__ __device__ constexpr int batch_lds_bytes(int bd) {
-  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd);
+  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd) + batch_comb_bytes(bd);
 }
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
-// The per-vertex bitmap: the last batch_far_bits_bytes(bd) of the dynamic LDS.
+// The per-vertex bitmap: batch_far_bits_bytes(bd) after the low-degree queues.
 __device__ __forceinline__ lds_u32* batch_bits(int* lds) {
   return (lds_u32*)(reinterpret_cast<uint32_t*>(lds) +
                     (frontier_lds_bytes((int)blockDim.x) + 2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
+}
+// The three combine tables: the last batch_comb_bytes(bd) of the dynamic LDS.
+__device__ __forceinline__ lds_u64* batch_comb(int* lds) {
+  return (lds_u64*)(batch_bits(lds) + batch_far_bits_bytes((int)blockDim.x) / 4);
+}
+
+__device__ __forceinline__ unsigned long long comb_word(int u, uint32_t alt, int cand) {
+  return ((unsigned long long)alt << 32) | ((unsigned long long)(uint32_t)u << 10) | (uint32_t)cand;
+}
+// Slot of target u (< 2^22) in a combine table, where its word is folded in
+// by a minimum: the first candidate of u claims an empty slot, later ones
+// find u there. -1 only if the table were full, which a pass cannot make it.
+__device__ __forceinline__ int comb_insert(lds_u64* t, int u, unsigned long long mine) {
+  int h = (int)(((uint32_t)u * 2654435761u) >> (32 - kCombBits));
+  for (int probe = 0; probe < kCombSlots; probe++) {
+    unsigned long long old = kCombEmpty;
+    if (__hip_atomic_compare_exchange_strong(&t[h], &old, mine, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return h;
+    if (((uint32_t)old >> 10) == (uint32_t)u) {
+      __hip_atomic_fetch_min(&t[h], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      return h;
+    }
+    h = (h + 1) & (kCombSlots - 1);
+  }
+  return -1;
+}
+
+// Workgroup barrier that orders LDS only: memory loads stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 struct BatchJob {
@@ -281,7 +331,16 @@ struct LowQueues {
 // vertex, word k = (target << 8) | weight of its k-th edge, kEllNone past its
 // degree -- so the K threads of an entry load one 4K-byte row together, an
 // entry needs no row_info, and the second round trip is the atomic alone.
-template <bool SSSP, int K, int U, bool ELL>
+// COMB (SSSP on packed rows with the LDS bitmap): no atomic on dist at all.
+// A pass loads each candidate target's distance with a plain load and folds
+// its candidates per target in an LDS combine table (comb_insert); after an
+// LDS-only barrier the one winner of each slot, if below the loaded
+// distance, stores it and queues the target. The workgroup is the only
+// writer of the graph's distances, so its L2 holds them and the second round
+// trip is an L2 hit instead of an atomic beyond the L2. A level of several
+// passes orders one pass's stores before the next pass's loads with a full
+// barrier.
+template <bool SSSP, int K, int U, bool ELL, bool COMB = false>
 __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int* lds, int* s_cnt,
                         int* iters_out, bool lds_flags, const uint32_t* ell) {
   const bool near_far = SSSP && nf.delta > 0;
@@ -301,6 +360,9 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   if (bits || vis)
     for (int w = threadIdx.x; w < (g.n + 31) / 32; w += blockDim.x)
       fbits[w] = vis && w == (src >> 5) ? 1u << (src & 31) : 0u;
+  lds_u64* comb = batch_comb(lds);
+  if (COMB)
+    for (int i = threadIdx.x; i < 3 * kCombSlots; i += blockDim.x) comb[i] = kCombEmpty;
   if (threadIdx.x == 0) {
     lq.store(0, 0, src, ELL ? 0u : row_info(g, src), 0u, SSSP, !ELL);
     s_cnt[0] = 1;
@@ -318,6 +380,10 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   int level = 0, fi = 0;
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   uint32_t nrel = 0;  // edges this thread relaxed (live entries' edges)
+  int tcur = 0;       // COMB: this pass's table
+  int clr[U];         // COMB: slot this thread won in the previous pass (cleared after this pass's barrier)
+#pragma unroll
+  for (int x = 0; x < U; x++) clr[x] = -1;
   while (true) {
     for (;; level++) {
       const int in_n = __builtin_amdgcn_readfirstlane(s_cnt[level % 3]);
@@ -331,9 +397,11 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         // slot x holds entries of this wave only while a + x per + its first
         // slot < in_n: a scalar test, so a wave runs no code for empty slots
         // (most levels fill one slot, and only some waves of it) -- issue
-        // time, not memory, bounds a narrow level
+        // time, not memory, bounds a narrow level. COMB passes have barriers:
+        // every wave runs them, and the per-slot tests below skip the rest.
         const int wslot = __builtin_amdgcn_readfirstlane((int)threadIdx.x & ~63) / K;
-        if (a + wslot >= in_n) continue;
+        if (COMB && a > 0) __syncthreads();  // the last pass's stores before this pass's loads
+        if (!COMB && a + wslot >= in_n) continue;
         bool lv[U];
 #pragma unroll
         for (int x = 0; x < U; x++) lv[x] = a + x * per + wslot < in_n;
@@ -375,8 +443,10 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         // Only the memory operations sit under each slot's condition; what
         // reads their results comes after every slot has issued.
         uint32_t alt[U], old[U], r0[U], r1[U];
+        int hs[U];  // COMB: the target's slot in this pass's table
 #pragma unroll
         for (int x = 0; x < U; x++) {
+          hs[x] = -1;
           const bool live = ok[x] && (!SSSP || dv[x] == du[x]);
           nrel += live ? 1u : 0u;
           alt[x] = SSSP ? du[x] + w[x] : next;
@@ -393,7 +463,12 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             r1[x] = (uint32_t)g.row_ptr[uu + 1];
           }
           if (live) {
-            if (vis) {
+            if (COMB) {
+              // the target's distance (plain load, in flight across the
+              // barrier) and the candidate into the combine table
+              old[x] = s.dist[u[x]];
+              hs[x] = comb_insert(comb + tcur * kCombSlots, u[x], comb_word(u[x], alt[x], x * bd + (int)threadIdx.x));
+            } else if (vis) {
               // BFS with visited bits: the first to set u's bit discovers it
               const uint32_t bit = 1u << (u[x] & 31);
               old[x] = (__hip_atomic_fetch_or(&fbits[u[x] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
@@ -406,6 +481,27 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
             }
           }
         }
+        bool won[U];
+#pragma unroll
+        for (int x = 0; x < U; x++) won[x] = false;
+        if constexpr (COMB) {
+          // every candidate is in the table; the slot's word is the minimum
+          // over its target's candidates, and the candidate whose word it is
+          // wins it (the thread bits break ties)
+          lds_barrier();
+          lds_u64* tb = comb + tcur * kCombSlots;
+          lds_u64* tp = comb + (tcur == 0 ? 2 : tcur - 1) * kCombSlots;
+#pragma unroll
+          for (int x = 0; x < U; x++) {
+            if (clr[x] >= 0) tp[clr[x]] = kCombEmpty;
+            clr[x] = -1;
+            if (hs[x] >= 0) {
+              won[x] = tb[hs[x]] == comb_word(u[x], alt[x], x * bd + (int)threadIdx.x);
+              if (won[x]) clr[x] = hs[x];
+            }
+          }
+          tcur = tcur == 2 ? 0 : tcur + 1;
+        }
 #pragma unroll
         for (int x = 0; x < U; x++) {
           opaque(old[x]);
@@ -417,11 +513,12 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
 #pragma unroll
         for (int x = 0; x < U; x++) {
           ru[x] = (r0[x] << 3) | (r1[x] - r0[x]);
-          const bool better = SSSP ? alt[x] < old[x] : old[x] == kInf;
+          const bool better = COMB ? won[x] && alt[x] < old[x] : SSSP ? alt[x] < old[x] : old[x] == kInf;
           want[x] = better && (!SSSP || alt[x] < thr);
           far_cand[x] = SSSP && better && alt[x] >= thr;
           // the discoverer writes the level: nothing reads dist during a BFS
           if (vis && want[x]) s.dist[u[x]] = alt[x];
+          if (COMB && better) s.dist[u[x]] = alt[x];
         }
 #pragma unroll
         for (int x = 0; x < U; x++) {
@@ -613,7 +710,11 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
       if (jb.maxdeg <= kLowDegree && jb.m < (1 << 29)) {
         if (b.low_ell && jb.e_off >= 0) {
           const uint32_t* ell = b.ell + jb.e_off;
-          if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
+          // the combine tables hold a pass's candidates at most half full
+          // and candidate ids fit 10 bits
+          if (sssp && b.low_ell == 2 && lds_flags && nf.delta && (int)blockDim.x * kLowSlots <= 1024)
+            run_low<true, kLowDegree, kLowSlots, true, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], true, ell);
+          else if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
           else run_low<false, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
         } else {
           if (sssp)
@@ -940,7 +1041,7 @@ int enqueue_batched(wgm_batch* b, const wgm_batch_config& cfg, int grid, hipStre
   if (rc) return rc;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, d);
   d.far_bits = cfg.far_flags_lds ? 1 : 0;
-  d.low_ell = cfg.low_ell ? 1 : 0;
+  d.low_ell = cfg.low_ell == 2 ? 2 : cfg.low_ell ? 1 : 0;
   hipLaunchKernelGGL(k_batched<0>, dim3(grid), dim3(cfg.wg_size), batch_lds_bytes(cfg.wg_size), st, rt->ctl, d);
   return enqueue_collect(b, st, false);  // k_batched sums each job's reached edges itself
 }
@@ -1246,7 +1347,7 @@ int wgm_batch_run_relaunch(wgm_runtime* rt, wgm_batch* b, const wgm_batch_config
   for (int k = 0; k < b->count; k++) {
     BatchDev one = d;
     one.far_bits = cfg.far_flags_lds ? 1 : 0;
-    one.low_ell = cfg.low_ell ? 1 : 0;
+    one.low_ell = cfg.low_ell == 2 ? 2 : cfg.low_ell ? 1 : 0;
     one.jobs = b->d_jobs_one + k;  // same job, tickets numbered from 0
     one.njobs = 1;
     one.total_tickets = b->jobs[k].team;
@@ -1413,7 +1514,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
   start.delta_high_degree = 12;
   start.sssp_team_edges = 786432;
   start.far_flags_lds = 1;
-  start.low_ell = 1;
+  start.low_ell = 2;
   // 1. launch shape: workgroup size x workgroups per CU
   const int wgs[] = {64, 128, 256, 512, 1024};
   const int per_cu[] = {1, 2, 4, 8};
@@ -1466,7 +1567,7 @@ int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_co
     }
   }
   // 3. the single-workgroup step's per-vertex LDS bits and packed rows
-  for (int flags = 0; flags < 4; flags++) {
+  for (int flags = 0; flags < 6; flags++) {
     wgm_batch_config c = best;
     c.far_flags_lds = flags & 1;
     c.low_ell = flags >> 1;

@@ -703,7 +703,11 @@ typedef struct wgm_batch_config {
                                 an atomic minimum on the distance per edge) */
   int32_t low_ell;           /* 1: a low-degree query on one workgroup reads its graph from packed rows,
                                 4 words per vertex of (target << 8) | weight, built at create for graphs
-                                of out-degree <= 4, fewer than 2^24 - 1 vertices and weights in [0, 255] */
+                                of out-degree <= 4, fewer than 2^24 - 1 vertices and weights in [0, 255];
+                                2: as 1, and an SSSP query among them that also has the LDS bitmap
+                                (far_flags_lds, near-far bands on) updates its distances with plain loads
+                                and stores that stay in its XCD's L2, its candidates folded per target in
+                                LDS hash tables first, instead of atomics (performed beyond the L2) */
 } wgm_batch_config;
 
 typedef struct wgm_batch_stats {
@@ -741,10 +745,10 @@ int wgm_batch_result(wgm_batch* b, int32_t index, uint32_t* dist_out, int32_t* i
 int wgm_batch_job_times(const wgm_batch* b, double* start_us, double* end_us);
 /* Search: wg_size {64..1024} x wgs_per_cu {1,2,4,8} from a start suited to
  * mixed batches (team_edges 2^20, sssp_team_edges 786432, bands 512 / 12,
- * LDS bits and packed rows on), then one knob at a time at the best so far:
+ * LDS bits on, packed rows with L2 distances (low_ell 2)), then one knob at a time at the best so far:
  * chunk {32..4096}; twice over team_edges {2^14..2^22}, sssp_team_edges
  * {0, 2^16..2^20}, the SSSP band {0, 256..4096} and the band of graphs with an
- * out-degree above 4 {0, 4..128}; then far_flags_lds x low_ell {0,1}^2.
+ * out-degree above 4 {0, 4..128}; then far_flags_lds {0,1} x low_ell {0,1,2}.
  * Each candidate runs `reps` times; returns the fastest configuration. */
 int wgm_batch_autotune(wgm_runtime* rt, wgm_batch* b, int32_t reps, wgm_batch_config* best,
                        wgm_batch_stats* best_stats);

@@ -79,7 +79,7 @@ def test_batched_autotune_returns_an_exact_config(rt, small_batch):
     assert cfg["delta"] in (0, 256, 512, 1024, 2048, 4096)
     assert cfg["delta_high_degree"] in (0, 4, 8, 12, 16, 32, 128)
     assert cfg["sssp_team_edges"] in (0, 1 << 16, 1 << 18, 1 << 19, 786432, 1 << 20)
-    assert cfg["far_flags_lds"] in (0, 1) and cfg["low_ell"] in (0, 1)
+    assert cfg["far_flags_lds"] in (0, 1) and cfg["low_ell"] in (0, 1, 2)
     assert st["wall_ms"] > 0
     b.run(best)
     _check_all(b, graphs, refs)
@@ -174,9 +174,30 @@ def test_batched_packed_rows(rt, small_batch, wg, delta):
     alternating with the CSR step on the same batch."""
     graphs, queries, refs = small_batch
     b = B.Batch(rt, graphs, queries)
-    for ell in (1, 0, 1):
+    for ell in (1, 0, 2, 1):
         b.run(wg_size=wg, wgs_per_cu=1, team_edges=512, max_team=8, delta=delta, delta_high_degree=8,
               far_flags_lds=1, low_ell=ell)
+        _check_all(b, graphs, refs)
+    b.close()
+
+
+@pytest.mark.parametrize("delta", [512, 4096, 1])
+def test_batched_lds_combine(rt, delta):
+    """Low-degree SSSP on one 1024-thread workgroup with its distances in the
+    L2 (low_ell 2): plain loads and stores, candidates folded per target in
+    the LDS combine tables. Wide bands give levels of several passes (the
+    barrier between passes), unit weights give ties (one winner per slot).
+    Exact against the oracle, alternating with the atomic step."""
+    from gpu_workgroup_managment_amd import graphs as G
+    graphs = [G.add_weights(G.grid(300, 300), seed=11), G.grid(200, 200), G.add_weights(G.grid(64, 1000), seed=12),
+              G.grid(120, 90)]
+    graphs[1].wt = np.ones(graphs[1].m, np.int32)
+    queries = np.array([L.QUERY_SSSP, L.QUERY_SSSP, L.QUERY_SSSP, L.QUERY_BFS], np.int32)
+    refs = [oracle.sssp(g.row_ptr, g.col, g.wt, 0) for g in graphs[:3]] + \
+        [oracle.bfs(graphs[3].row_ptr, graphs[3].col, 0)[0]]
+    b = B.Batch(rt, graphs, queries)
+    for ell in (2, 1, 2):
+        b.run(wg_size=1024, wgs_per_cu=1, delta=delta, far_flags_lds=1, low_ell=ell)
         _check_all(b, graphs, refs)
     b.close()
 

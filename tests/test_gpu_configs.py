@@ -180,6 +180,8 @@ def test_config_batched512(rt, batch512):
     _check_batch(b, graphs, refs)
     st = b.run(B.config(use_graph=True, far_flags_lds=1, low_ell=1, **BATCH_SHAPE))
     _check_batch(b, graphs, refs)
+    st = b.run(B.config(use_graph=True, far_flags_lds=1, low_ell=2, **BATCH_SHAPE))
+    _check_batch(b, graphs, refs)
     st = b.run_relaunch(B.config(**dict(BATCH_SHAPE, team_edges=1 << 16)))
     assert st["launches"] == len(graphs)
     _check_batch(b, graphs, refs)
