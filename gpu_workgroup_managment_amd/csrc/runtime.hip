@@ -47,24 +47,28 @@ __global__ void k_reset_ctl(Ctl c, int flag_lines, int gid_entries) {
 // --------------------------------------------------------- barrier bench
 // os_freq (src/test_suite/os_freq/device/os_freq.cl:8-46): iterations x 10
 // barriers. With `check`, every workgroup counts its arrivals in one word
-// and after barrier b verifies (b+1)*n <= arrivals <= (b+2)*n.
-__global__ void __launch_bounds__(1024) k_barrier_bench(Ctl c, int iterations, int variant, int check,
-                                                        int* errors, int* arrivals) {
+// and after barrier b verifies (b+1)*n <= arrivals <= (b+2)*n. Every wait has
+// a budget of kBenchSpins spins: a barrier that does not complete raises
+// *abort and the kernel ends (its results are then void) instead of hanging.
+constexpr unsigned kBenchSpins = 1u << 22;
+__global__ void __launch_bounds__(1024, 8) k_barrier_bench(Ctl c, int iterations, int variant, int check,
+                                                        int* errors, int* arrivals, int* abort) {
   __shared__ int local_sense;
   Team t;
   if (!discover_team(c, &t) || c.im_exit) return;
   const int id = t.id, n = t.n;
+  const SpinBounded sp{abort, kBenchSpins};
   if (id == 0 && threadIdx.x == 0) st_relaxed(c.bar_groups, n);
   if (threadIdx.x == 0) local_sense = 0;
-  global_barrier_disc(c, id, n);
+  global_barrier_disc(c, id, n, sp);
   clock_start(c, id);
   int b = 0;
   for (int i = 0; i < iterations; i++) {
     for (int j = 0; j < 10; j++, b++) {
       if (check && threadIdx.x == 0) add_relaxed(arrivals, 1);
-      if (variant == 0) global_barrier_disc(c, id, n);
-      else if (variant == 2) xcd_barrier(c, t);
-      else global_barrier_sense(c.sense_counter, c.sense_flag, c.bar_groups, &local_sense);
+      if (variant == 0) global_barrier_disc(c, id, n, sp);
+      else if (variant == 2) xcd_barrier<true>(c, t, sp);
+      else global_barrier_sense(c.sense_counter, c.sense_flag, c.bar_groups, &local_sense, sp);
       if (check && threadIdx.x == 0) {
         const int a = ld_relaxed(arrivals);
         if (a < (b + 1) * n || a > (b + 2) * n) add_relaxed(errors, 1);
@@ -441,7 +445,7 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size, int
     case WGM_K_DISCOVERY:
     case WGM_K_BARRIER_BENCH:
       hipLaunchKernelGGL(k_barrier_bench, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, 0, 0, 0,
-                         (int*)nullptr, (int*)nullptr);
+                         (int*)nullptr, (int*)nullptr, (int*)nullptr);
       break;
     default: {
       int rc2 = wgm_app_probe(rt, kernel, grid, wg_size);
@@ -469,7 +473,7 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
   int rc = wgm_resolve_grid(rt, WGM_K_BARRIER_BENCH, wg_size, num_wgs, &grid);
   if (rc) return rc;
   int* d_aux = nullptr;
-  HIPCHK(rt, hipMalloc(&d_aux, 2 * sizeof(int)));
+  HIPCHK(rt, hipMalloc(&d_aux, 3 * sizeof(int)));  // errors, arrivals, abort
   if (variant == WGM_BARRIER_COOP) {
     // The grid is never above what the discovery probe found co-resident for
     // k_barrier_bench (which uses more registers and LDS than this kernel),
@@ -517,18 +521,22 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
     return WGM_OK;
   }
   auto run = [&](int check, int iters, wgm_barrier_result* r) -> int {
-    HIPCHK(rt, hipMemsetAsync(d_aux, 0, 2 * sizeof(int), rt->stream));
+    HIPCHK(rt, hipMemsetAsync(d_aux, 0, 3 * sizeof(int), rt->stream));
     int rc2 = wgm_reset_ctl(rt, 0, 0);
     if (rc2) return rc2;
     HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
     hipLaunchKernelGGL(k_barrier_bench, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, iters, variant, check,
-                       d_aux, d_aux + 1);
+                       d_aux, d_aux + 1, d_aux + 2);
     HIPCHK(rt, hipGetLastError());
     wgm_run_stats st{};
     rc2 = wgm_finish_launch(rt, &st);
     if (rc2) return rc2;
-    int errs = 0;
-    HIPCHK(rt, hipMemcpy(&errs, d_aux, sizeof(int), hipMemcpyDeviceToHost));
+    int aux[3] = {0, 0, 0};
+    HIPCHK(rt, hipMemcpy(aux, d_aux, sizeof(aux), hipMemcpyDeviceToHost));
+    if (aux[2])
+      return fail(rt, WGM_ERR_TIMEOUT, "barrier variant %d: a wait exceeded %u spins (%d participants)", variant,
+                  kBenchSpins, st.participating);
+    const int errs = aux[0];
     r->participating = st.participating;
     r->barriers = iters * 10;
     r->epoch_errors = errs;

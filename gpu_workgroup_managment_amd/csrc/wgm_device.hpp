@@ -224,21 +224,47 @@ __device__ __forceinline__ bool discover_team(const Ctl& c, Team* t) {
   return true;
 }
 
+// ------------------------------------------------------------ spin policies
+// How long a barrier waits. Production kernels wait as long as it takes
+// (SpinForever, which compiles to the bare loop). The barrier microbenchmark
+// gives every wait a budget of spins (SpinBounded): a workgroup that exhausts
+// it raises the abort line, every other wait sees the line within 64 spins
+// and gives up too, so a barrier that never completes ends the kernel with an
+// error instead of hanging it.
+struct SpinForever {
+  __device__ __forceinline__ bool go(unsigned&) const { return true; }
+};
+struct SpinBounded {
+  int* abort;       // device line, nonzero once any wait gave up
+  unsigned budget;  // spins per wait
+  __device__ __forceinline__ bool go(unsigned& k) const {
+    ++k;
+    if ((k & 63u) == 0u && ld_relaxed(abort)) return false;
+    if (k >= budget) {
+      st_relaxed(abort, 1);
+      return false;
+    }
+    return true;
+  }
+};
+
 // ------------------------------------------------------------ flag barrier
 // Master/slave barrier over discovered ids 0..n-1 (global_barrier_disc).
 // Slaves publish arrival in their own line and spin there; the master polls
 // up to eight peers per thread with all loads in flight at once, then
 // releases every slave. One agent-scope release fence before each batch of
 // relaxed flag stores and one acquire fence after each spin.
-__device__ __forceinline__ void master_collect(int* flags, int lo, int hi) {
+template <class Spin>
+__device__ __forceinline__ void master_collect(int* flags, int lo, int hi, const Spin& sp) {
   // wait for flags[p] == 1 for p in [lo, hi)
   const int tid = threadIdx.x, bd = blockDim.x;
+  unsigned spins = 0;
   if (hi - lo <= 8 * bd) {
     unsigned pending = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++)
       if (lo + tid + k * bd < hi) pending |= 1u << k;
-    while (pending) {
+    while (pending && sp.go(spins)) {
       int v[8];
 #pragma unroll
       for (int k = 0; k < 8; k++)
@@ -249,7 +275,7 @@ __device__ __forceinline__ void master_collect(int* flags, int lo, int hi) {
     }
   } else {
     for (int p = lo + tid; p < hi; p += bd)
-      while (ld_relaxed(&flags[p * kLine]) == 0) {
+      while (ld_relaxed(&flags[p * kLine]) == 0 && sp.go(spins)) {
       }
   }
 }
@@ -259,29 +285,32 @@ __device__ __forceinline__ void master_release(int* flags, int lo, int hi) {
   for (int p = lo + (int)threadIdx.x; p < hi; p += blockDim.x) st_relaxed(&flags[p * kLine], 0);
 }
 
-__device__ __forceinline__ void slave_arrive_wait(int* flags, int id) {
+template <class Spin>
+__device__ __forceinline__ void slave_arrive_wait(int* flags, int id, const Spin& sp) {
   int* f = &flags[id * kLine];
   fence_release();
   st_relaxed(f, 1);
-  while (ld_relaxed(f) != 0) {
+  unsigned spins = 0;
+  while (ld_relaxed(f) != 0 && sp.go(spins)) {
   }
   fence_acquire();
 }
 
-__device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n) {
+template <class Spin = SpinForever>
+__device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n, const Spin& sp = Spin()) {
   drain_vmem();
   if (n <= 1) {
     __syncthreads();
     return;
   }
   if (id == 0) {
-    master_collect(c.flags, 1, n);
+    master_collect(c.flags, 1, n, sp);
     fence_acquire();
     __syncthreads();
     master_release(c.flags, 1, n);
   } else {
     __syncthreads();
-    if (threadIdx.x == 0) slave_arrive_wait(c.flags, id);
+    if (threadIdx.x == 0) slave_arrive_wait(c.flags, id, sp);
     __syncthreads();
   }
 }
@@ -304,8 +333,8 @@ __device__ __forceinline__ void global_barrier_disc(const Ctl& c, int id, int n)
 // MAINT = false drops the write-back and the acquires: for callers whose
 // shared state goes only through agent-scope atomics (run_team), where the
 // barrier need only order arrivals after this workgroup's drained accesses.
-template <bool MAINT = true>
-__device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
+template <bool MAINT = true, class Spin = SpinForever>
+__device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t, const Spin& sp = Spin()) {
   drain_vmem();
   if (t.n <= 1) {
     __syncthreads();
@@ -314,19 +343,20 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
   t.epoch++;
   const int arrive = 2 * t.epoch - 1, release = 2 * t.epoch;
   const int* members = c.xcd_members + t.xcc * c.max_groups;
+  unsigned spins = 0;
   __syncthreads();
   if (t.xrank != 0) {
     if (threadIdx.x == 0) {
       int* f = &c.flags[t.id * kLine];
       st_relaxed(f, arrive);
-      while (ld_relaxed(f) != release) {
+      while (ld_relaxed(f) != release && sp.go(spins)) {
       }
       if (MAINT) fence_acquire();
     }
   } else {
     for (int r = 1 + (int)threadIdx.x; r < t.xm; r += blockDim.x) {
       const int* f = &c.flags[members[r] * kLine];
-      while (ld_relaxed(f) != arrive) {
+      while (ld_relaxed(f) != arrive && sp.go(spins)) {
       }
     }
     __syncthreads();
@@ -335,7 +365,7 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
       st_relaxed(&c.xflags[t.xcc * kLine], t.epoch);
     }
     if (threadIdx.x < kMaxXcd && ld_relaxed(&c.xcd_count[threadIdx.x * kLine]) > 0) {
-      while (ld_relaxed(&c.xflags[threadIdx.x * kLine]) < t.epoch) {
+      while (ld_relaxed(&c.xflags[threadIdx.x * kLine]) < t.epoch && sp.go(spins)) {
       }
     }
     __syncthreads();
@@ -349,14 +379,16 @@ __device__ __forceinline__ void xcd_barrier(const Ctl& c, Team& t) {
 // -------------------------------------------------- sense-reversal barrier
 // Counter + sense barrier whose group count is re-read on every spin
 // (global_barrier_robust_to_resizing); `*groups` may change between calls.
+template <class Spin = SpinForever>
 __device__ __forceinline__ void global_barrier_sense(int* counter, int* sense_word, const int* groups,
-                                                     int* local_sense) {
+                                                     int* local_sense, const Spin& sp = Spin()) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const int s = !(*local_sense);
     *local_sense = s;
+    unsigned spins = 0;
     if (add_acq_rel(counter, 1) == 0) {
-      while (true) {
+      while (sp.go(spins)) {
         const int arrived = ld_acquire(counter);
         const int n = ld_relaxed(groups);
         if (arrived == n) {
@@ -366,7 +398,7 @@ __device__ __forceinline__ void global_barrier_sense(int* counter, int* sense_wo
         }
       }
     } else {
-      while (ld_relaxed(sense_word) != s) {
+      while (ld_relaxed(sense_word) != s && sp.go(spins)) {
       }
       fence_acquire();
     }
