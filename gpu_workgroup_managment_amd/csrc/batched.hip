@@ -66,20 +66,31 @@ constexpr int32_t kEllMaxN = (1 << 24) - 1;
 // 1024-thread workgroup updates its distances with plain loads and stores,
 // which stay in its XCD's L2, instead of atomics, which are all performed
 // beyond it. Each pass first folds its candidates per target in an LDS hash
-// table of kCombSlots 64-bit words, (distance << 32) | (target << 10) |
-// candidate (the thread, so that words are unique and the minimum has one
-// owner), open addressing with linear probes. A pass has at most blockDim.x
-// <= 1024 candidates, so a table is at most half full and every probe
-// sequence ends.
+// table of kCombSlots 64-bit words, (distance << 32) | (target << 11) |
+// candidate (slot and thread, so that words are unique and the minimum has
+// one owner), open addressing with linear probes. A pass takes kCombU
+// entries per 4-lane group, at most kCombU * blockDim.x <= 2048 candidates,
+// so a table is at most half full and every probe sequence ends.
 // Three tables serve consecutive passes in turn: the winner of a slot clears
 // it after the next pass's barrier, when every reader of the slot is past it,
 // and the pass after that (the slot's next user) starts after the clear.
-constexpr int kCombBits = 11;
+// Passes are the cost of a level (each is two memory round trips and two
+// barriers): two entries per group halve the passes of the levels wider than
+// blockDim.x / 4 entries.
+constexpr int kCombBits = 12;
 constexpr int kCombSlots = 1 << kCombBits;
+constexpr int kCombU = 2;
 constexpr unsigned long long kCombEmpty = ~0ull;
-__host__ __device__ constexpr int batch_comb_bytes(int bd) { return bd >= 1024 ? 3 * kCombSlots * 8 : 0; }
+// Its own LDS layout: two frontier queues of kCombCap (vertex, tag) entries,
+// the far bitmap, the three tables (152 KB at 1024 threads).
+constexpr int kCombCap = 1536;
+__host__ __device__ constexpr int comb_lds_bytes(int bd) {
+  return bd >= 1024 ? 2 * 2 * 4 * kCombCap + 4 * kFarBitWords + 3 * kCombSlots * 8 : 0;
+}
 __host__ __device__ constexpr int batch_lds_bytes(int bd) {
-  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd) + batch_comb_bytes(bd);
+  return frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd) > comb_lds_bytes(bd)
+             ? frontier_lds_bytes(bd) + 2 * 3 * 4 * low_lds_cap(bd) + batch_far_bits_bytes(bd)
+             : comb_lds_bytes(bd);
 }
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
@@ -88,15 +99,14 @@ __device__ __forceinline__ lds_u32* batch_bits(int* lds) {
   return (lds_u32*)(reinterpret_cast<uint32_t*>(lds) +
                     (frontier_lds_bytes((int)blockDim.x) + 2 * 3 * 4 * low_lds_cap((int)blockDim.x)) / 4);
 }
-// The three combine tables: the last batch_comb_bytes(bd) of the dynamic LDS.
-__device__ __forceinline__ lds_u64* batch_comb(int* lds) {
-  return (lds_u64*)(batch_bits(lds) + batch_far_bits_bytes((int)blockDim.x) / 4);
-}
+// The combine layout's far bitmap and tables.
+__device__ __forceinline__ lds_u32* comb_bits(int* lds) { return (lds_u32*)lds + 2 * 2 * kCombCap; }
+__device__ __forceinline__ lds_u64* batch_comb(int* lds) { return (lds_u64*)(comb_bits(lds) + kFarBitWords); }
 
 __device__ __forceinline__ unsigned long long comb_word(int u, uint32_t alt, int cand) {
-  return ((unsigned long long)alt << 32) | ((unsigned long long)(uint32_t)u << 10) | (uint32_t)cand;
+  return ((unsigned long long)alt << 32) | ((unsigned long long)(uint32_t)u << 11) | (uint32_t)cand;
 }
-// Slot of target u (< 2^22) in a combine table, where its word is folded in
+// Slot of target u (< 2^21) in a combine table, where its word is folded in
 // by a minimum: the first candidate of u claims an empty slot, later ones
 // find u there. -1 only if the table were full, which a pass cannot make it.
 __device__ __forceinline__ int comb_insert(lds_u64* t, int u, unsigned long long mine) {
@@ -106,7 +116,7 @@ __device__ __forceinline__ int comb_insert(lds_u64* t, int u, unsigned long long
     if (__hip_atomic_compare_exchange_strong(&t[h], &old, mine, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_WORKGROUP))
       return h;
-    if (((uint32_t)old >> 10) == (uint32_t)u) {
+    if (((uint32_t)old >> 11) == (uint32_t)u) {
       __hip_atomic_fetch_min(&t[h], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       return h;
     }
@@ -282,18 +292,19 @@ __device__ __forceinline__ uint32_t row_info(const GraphDev& g, int v) {
 // counters.
 
 struct LowQueues {
-  lds_u32* lds;  // [queue][vertex, row info, tag][cap]
+  lds_u32* lds;  // [queue][vertex, row info, tag][cap]; F = 2: [queue][vertex, tag][cap] (no row info)
   int cap;
   const FrontierDev* s;
+  int F;
   // rows: the entry carries row_info (CSR rows; packed rows need none)
   __device__ __forceinline__ void load(int p, int i, int& v, uint32_t& ri, uint32_t& du, bool tag,
                                        bool rows = true) const {
     ri = 0u;
     if (i < cap) {
-      const lds_u32* b = lds + p * 3 * cap;
+      const lds_u32* b = lds + p * F * cap;
       v = (int)b[i];
       if (rows) ri = b[cap + i];
-      du = tag ? b[2 * cap + i] : 0u;
+      du = tag ? b[(F - 1) * cap + i] : 0u;
     } else {
       v = (p ? s->q1 : s->q0)[i];
       if (rows) ri = (p ? s->qr1 : s->qr0)[i];
@@ -303,10 +314,10 @@ struct LowQueues {
   __device__ __forceinline__ void store(int p, int i, int v, uint32_t ri, uint32_t du, bool tag,
                                         bool rows = true) const {
     if (i < cap) {
-      lds_u32* b = lds + p * 3 * cap;
+      lds_u32* b = lds + p * F * cap;
       b[i] = (uint32_t)v;
       if (rows) b[cap + i] = ri;
-      if (tag) b[2 * cap + i] = du;
+      if (tag) b[(F - 1) * cap + i] = du;
     } else {
       (p ? s->q1 : s->q0)[i] = v;
       if (rows) (p ? s->qr1 : s->qr0)[i] = ri;
@@ -345,12 +356,13 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
                         int* iters_out, bool lds_flags, const uint32_t* ell) {
   const bool near_far = SSSP && nf.delta > 0;
   __shared__ int s_far[3];  // far counts of q[0], q[1]; band minimum
-  const LowQueues lq{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
-                     low_lds_cap((int)blockDim.x), &s};
+  const LowQueues lq = COMB ? LowQueues{(lds_u32*)lds, kCombCap, &s, 2}
+                            : LowQueues{(lds_u32*)(reinterpret_cast<uint32_t*>(lds) + frontier_lds_bytes((int)blockDim.x) / 4),
+                                        low_lds_cap((int)blockDim.x), &s, 3};
   // one bit per vertex in LDS (lds_flags): SSSP's far flags (else a word per
   // vertex in HBM); BFS's visited marks, which then replace the atomic on
   // dist (else the atomic minimum decides the discovery)
-  lds_u32* fbits = batch_bits(lds);
+  lds_u32* fbits = COMB ? comb_bits(lds) : batch_bits(lds);
   const bool bits = near_far && lds_flags;
   const bool vis = !SSSP && lds_flags;
   for (int v = threadIdx.x; v < g.n; v += blockDim.x) {
@@ -381,6 +393,12 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   uint32_t nrel = 0;  // edges this thread relaxed (live entries' edges)
   int tcur = 0;       // COMB: this pass's table
+  // COMB: a queued target's packed row is loaded as it is queued, so that the
+  // next level finds it in the cache; the loads complete by the level's
+  // barrier and their values are only folded into pfacc (never read)
+  uint32_t pfv[U], pfacc = 0;
+#pragma unroll
+  for (int x = 0; x < U; x++) pfv[x] = 0u;
   int clr[U];         // COMB: slot this thread won in the previous pass (cleared after this pass's barrier)
 #pragma unroll
   for (int x = 0; x < U; x++) clr[x] = -1;
@@ -400,7 +418,14 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         // time, not memory, bounds a narrow level. COMB passes have barriers:
         // every wave runs them, and the per-slot tests below skip the rest.
         const int wslot = __builtin_amdgcn_readfirstlane((int)threadIdx.x & ~63) / K;
-        if (COMB && a > 0) __syncthreads();  // the last pass's stores before this pass's loads
+        if (COMB && a > 0) {
+          __syncthreads();  // the last pass's stores before this pass's loads
+#pragma unroll
+          for (int x = 0; x < U; x++) {
+            pfacc ^= pfv[x];
+            pfv[x] = 0u;
+          }
+        }
         if (!COMB && a + wslot >= in_n) continue;
         bool lv[U];
 #pragma unroll
@@ -519,6 +544,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
           // the discoverer writes the level: nothing reads dist during a BFS
           if (vis && want[x]) s.dist[u[x]] = alt[x];
           if (COMB && better) s.dist[u[x]] = alt[x];
+          if (COMB && want[x]) pfv[x] = ell[(size_t)u[x] * K];
         }
 #pragma unroll
         for (int x = 0; x < U; x++) {
@@ -549,6 +575,11 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
         }
       }
       __syncthreads();
+#pragma unroll
+      for (int x = 0; x < U; x++) {
+        pfacc ^= pfv[x];
+        pfv[x] = 0u;
+      }
     }
     if (!near_far) break;
     // uniform: read after the level's barrier (scalar, so the branches
@@ -628,6 +659,7 @@ __device__ void run_low(const GraphDev& g, const FrontierDev& s, const NearFar& 
     fi = 1 - fi;
     __syncthreads();
   }
+  asm volatile("" ::"v"(pfacc));  // keeps the row loads (their values are unused)
   for (int d = 32; d > 0; d >>= 1) nrel += __shfl_down(nrel, d, 64);
   if ((threadIdx.x & 63) == 0 && nrel && s.relaxed) atomicAdd(s.relaxed, (unsigned long long)nrel);
   if (threadIdx.x == 0) *iters_out = level;
@@ -711,9 +743,9 @@ __global__ void __launch_bounds__(1024) k_batched(Ctl c, BatchDev b) {
         if (b.low_ell && jb.e_off >= 0) {
           const uint32_t* ell = b.ell + jb.e_off;
           // the combine tables hold a pass's candidates at most half full
-          // and candidate ids fit 10 bits
-          if (sssp && b.low_ell == 2 && lds_flags && nf.delta && (int)blockDim.x * kLowSlots <= 1024)
-            run_low<true, kLowDegree, kLowSlots, true, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], true, ell);
+          // and candidate ids fit 11 bits
+          if (sssp && b.low_ell == 2 && lds_flags && nf.delta && (int)blockDim.x * kCombU <= kCombSlots / 2)
+            run_low<true, kLowDegree, kCombU, true, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], true, ell);
           else if (sssp) run_low<true, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
           else run_low<false, kLowDegree, kLowSlots, true>(g, s, nf, jb.src, lds, s_cnt, &b.iters[j], lds_flags, ell);
         } else {
