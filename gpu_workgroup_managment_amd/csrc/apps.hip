@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, in
   standalone_retire(c);
 }
 
-__global__ void __launch_bounds__(1024) k_pagerank(Ctl c, GraphDev g, float* c0, float* c1, float* rank,
+__global__ void __launch_bounds__(1024, 8) k_pagerank(Ctl c, GraphDev g, float* c0, float* c1, float* rank,
                                                    float damping, float base, float r0, int iters,
                                                    int* iters_out) {
   Team t;
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(1024) k_pagerank(Ctl c, GraphDev g, float* c0,
   for (int it = 0; it < iters; it++) {
     const float* cin = (it & 1) ? c1 : c0;
     float* cout = (it & 1) ? c0 : c1;
-    pagerank_iter(g, cin, cout, rank, damping, base, it == iters - 1, lo, hi);
+    pagerank_iter<true>(g, cin, cout, rank, damping, base, it == iters - 1, lo, hi);
     xcd_barrier(c, t);
   }
   clock_end(c, t.id);

@@ -642,15 +642,57 @@ __device__ __forceinline__ void pagerank_init(const GraphDev& g, float* contrib,
   }
 }
 
+// GROUPED (the standalone kernel): kTopoGroup vertices per thread per pass,
+// their rows, then the first kTopoEdges edges of each, then those
+// neighbours' contributions, all in flight together (three round trips per
+// group instead of two per edge); each sum still adds its terms one by one in
+// edge order, so the result is the same bit for bit.
+template <bool GROUPED = false>
 __device__ __forceinline__ void pagerank_iter(const GraphDev& g, const float* cin, float* cout, float* rank,
                                               float damping, float base, bool last, int lo, int hi) {
-  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-    const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
-    float sum = 0.0f;
-    for (int e = beg; e < end; e++) sum = sum + cin[g.col[e]];
-    const float r = __builtin_fmaf(damping, sum, base);
-    if (last) rank[v] = r;
-    else cout[v] = (end > beg) ? r / (float)(end - beg) : 0.0f;
+  if (!GROUPED) {
+    for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+      const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
+      float sum = 0.0f;
+      for (int e = beg; e < end; e++) sum = sum + cin[g.col[e]];
+      const float r = __builtin_fmaf(damping, sum, base);
+      if (last) rank[v] = r;
+      else cout[v] = (end > beg) ? r / (float)(end - beg) : 0.0f;
+    }
+    return;
+  }
+  const int bd = blockDim.x;
+  for (int b0 = lo + (int)threadIdx.x; b0 < hi; b0 += kTopoGroup * bd) {
+    int v[kTopoGroup], beg[kTopoGroup], end[kTopoGroup];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      v[k] = b0 + k * bd;
+      const bool in = v[k] < hi;
+      beg[k] = in ? g.row_ptr[v[k]] : 0;
+      end[k] = in ? g.row_ptr[v[k] + 1] : 0;
+    }
+    int u[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) u[k][j] = beg[k] + j < end[k] ? g.col[beg[k] + j] : -1;
+    float cu[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) cu[k][j] = u[k][j] >= 0 ? cin[u[k][j]] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      if (v[k] >= hi) continue;
+      float sum = 0.0f;
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++)
+        if (u[k][j] >= 0) sum = sum + cu[k][j];
+      for (int e = beg[k] + kTopoEdges; e < end[k]; e++) sum = sum + cin[g.col[e]];
+      const float r = __builtin_fmaf(damping, sum, base);
+      if (last) rank[v[k]] = r;
+      else cout[v[k]] = (end[k] > beg[k]) ? r / (float)(end[k] - beg[k]) : 0.0f;
+    }
   }
 }
 
