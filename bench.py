@@ -348,6 +348,17 @@ def configs_leg(rt, reps=5, coop=False):
             bar[f"{name}_wg{wg}"] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
                                      "epoch_errors": r["epoch_errors"]}
     out["barrier_full_occupancy"] = bar
+    # the two-level barrier over a sweep of workgroup counts (1024 threads each)
+    sweep = {}
+    for nw in (64, 128, 256):
+        log(f"config barrier xcd wg 1024 x {nw}")
+        try:
+            r = rt.barrier_bench(1024, nw, 200, 2)
+            sweep[str(nw)] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
+                              "epoch_errors": r["epoch_errors"]}
+        except RuntimeError as e:
+            sweep[str(nw)] = {"error": str(e)}
+    out["barrier_xcd_wg1024_sweep"] = sweep
     log("config barrier: done")
     return out
 
