@@ -48,16 +48,17 @@ __global__ void k_reset_ctl(Ctl c, int flag_lines, int gid_entries) {
 // os_freq (src/test_suite/os_freq/device/os_freq.cl:8-46): iterations x 10
 // barriers. With `check`, every workgroup counts its arrivals in one word
 // and after barrier b verifies (b+1)*n <= arrivals <= (b+2)*n. Every wait has
-// a budget of kBenchSpins spins: a barrier that does not complete raises
-// *abort and the kernel ends (its results are then void) instead of hanging.
+// a budget of `budget` spins (kBenchSpins unless WGM_BARRIER_SPIN_BUDGET says
+// otherwise): a barrier that does not complete raises *abort and the kernel
+// ends (its results are then void) instead of hanging.
 constexpr unsigned kBenchSpins = 1u << 22;
 __global__ void __launch_bounds__(1024, 8) k_barrier_bench(Ctl c, int iterations, int variant, int check,
-                                                        int* errors, int* arrivals, int* abort) {
+                                                        int* errors, int* arrivals, int* abort, unsigned budget) {
   __shared__ int local_sense;
   Team t;
   if (!discover_team(c, &t) || c.im_exit) return;
   const int id = t.id, n = t.n;
-  const SpinBounded sp{abort, kBenchSpins};
+  const SpinBounded sp{abort, budget};
   if (id == 0 && threadIdx.x == 0) st_relaxed(c.bar_groups, n);
   if (threadIdx.x == 0) local_sense = 0;
   global_barrier_disc(c, id, n, sp);
@@ -445,7 +446,7 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size, int
     case WGM_K_DISCOVERY:
     case WGM_K_BARRIER_BENCH:
       hipLaunchKernelGGL(k_barrier_bench, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, 0, 0, 0,
-                         (int*)nullptr, (int*)nullptr, (int*)nullptr);
+                         (int*)nullptr, (int*)nullptr, (int*)nullptr, kBenchSpins);
       break;
     default: {
       int rc2 = wgm_app_probe(rt, kernel, grid, wg_size);
@@ -520,13 +521,16 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
     *out = timed;
     return WGM_OK;
   }
+  unsigned budget = kBenchSpins;
+  if (const char* b = std::getenv("WGM_BARRIER_SPIN_BUDGET"))
+    if (std::atol(b) >= 1) budget = (unsigned)std::atol(b);
   auto run = [&](int check, int iters, wgm_barrier_result* r) -> int {
     HIPCHK(rt, hipMemsetAsync(d_aux, 0, 3 * sizeof(int), rt->stream));
     int rc2 = wgm_reset_ctl(rt, 0, 0);
     if (rc2) return rc2;
     HIPCHK(rt, hipEventRecord(rt->ev0, rt->stream));
     hipLaunchKernelGGL(k_barrier_bench, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, iters, variant, check,
-                       d_aux, d_aux + 1, d_aux + 2);
+                       d_aux, d_aux + 1, d_aux + 2, budget);
     HIPCHK(rt, hipGetLastError());
     wgm_run_stats st{};
     rc2 = wgm_finish_launch(rt, &st);
@@ -535,7 +539,7 @@ int wgm_barrier_bench(wgm_runtime* rt, int32_t variant, int32_t wg_size, int32_t
     HIPCHK(rt, hipMemcpy(aux, d_aux, sizeof(aux), hipMemcpyDeviceToHost));
     if (aux[2])
       return fail(rt, WGM_ERR_TIMEOUT, "barrier variant %d: a wait exceeded %u spins (%d participants)", variant,
-                  kBenchSpins, st.participating);
+                  budget, st.participating);
     const int errs = aux[0];
     r->participating = st.participating;
     r->barriers = iters * 10;

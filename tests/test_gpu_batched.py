@@ -202,6 +202,20 @@ def test_batched_lds_combine(rt, delta):
     b.close()
 
 
+def test_batched_lds_combine_largest_bitmap(rt):
+    """A 512 x 512 grid SSSP (2^18 vertices, the most the LDS bitmap holds) on
+    the L2-distance step beside a 513 x 512 grid, one vertex row too many,
+    which takes the atomic step: both exact."""
+    from gpu_workgroup_managment_amd import graphs as G
+    graphs = [G.add_weights(G.grid(512, 512), seed=21), G.add_weights(G.grid(513, 512), seed=22)]
+    queries = np.array([L.QUERY_SSSP, L.QUERY_SSSP], np.int32)
+    refs = [oracle.sssp(g.row_ptr, g.col, g.wt, 0) for g in graphs]
+    b = B.Batch(rt, graphs, queries)
+    b.run(wg_size=1024, wgs_per_cu=1, delta=512, far_flags_lds=1, low_ell=2)
+    _check_all(b, graphs, refs)
+    b.close()
+
+
 def test_batched_packed_rows_skip_wide_weights(rt):
     """A low-degree graph with a weight above 255 keeps the CSR step (its
     weights do not fit the packed word); its neighbours in the batch take the

@@ -46,3 +46,18 @@ def test_barrier_epochs(rt, variant, wgs):
     print(r)
     assert r["epoch_errors"] == 0
     assert r["participating"] >= (wgs if wgs else 1)
+
+
+@pytest.mark.parametrize("variant", [L.BARRIER_FLAGS, L.BARRIER_XCD])
+def test_barrier_bench_spin_budget_ends_the_launch(rt, monkeypatch, variant):
+    """A barrier wait that runs out of its spin budget ends the launch with
+    WGM_ERR_TIMEOUT instead of hanging (forced from the host with a budget of
+    one spin, so every workgroup's waits give up and the kernel exits); the
+    next run with the default budget is exact."""
+    monkeypatch.setenv("WGM_BARRIER_SPIN_BUDGET", "1")
+    with pytest.raises(L.WgmError) as err:
+        rt.barrier_bench(wg_size=256, num_wgs=64, iterations=20, variant=variant)
+    assert err.value.code == -7
+    monkeypatch.delenv("WGM_BARRIER_SPIN_BUDGET")
+    r = rt.barrier_bench(wg_size=256, num_wgs=64, iterations=20, variant=variant)
+    assert r["epoch_errors"] == 0 and r["participating"] == 64
