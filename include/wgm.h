@@ -193,9 +193,10 @@ int wgm_discover_occupancy(wgm_runtime* rt, int32_t kernel, int32_t wg_size,
  * barriers in one launch. Replaces the os_freq test (kernel
  * src/test_suite/os_freq/device/os_freq.cl:8-46, host
  * src/test_suite/os_freq/host/os_freq.cpp:262-329). Every wait of the
- * FLAGS / SENSE / XCD variants has a budget of 2^22 spins: a barrier that
- * does not complete ends the launch and returns WGM_ERR_TIMEOUT. The COOP
- * variant's grid sync is HIP's and has no such bound. */
+ * FLAGS / SENSE / XCD variants has a budget of 2^22 spins (the environment
+ * variable WGM_BARRIER_SPIN_BUDGET, read at each call, replaces it): a barrier
+ * that does not complete ends the launch and returns WGM_ERR_TIMEOUT. The
+ * COOP variant's grid sync is HIP's and has no such bound. */
 #define WGM_BARRIER_FLAGS 0   /* flat master/slave flag barrier, one line per WG */
 #define WGM_BARRIER_SENSE 1   /* counter + sense reversal */
 #define WGM_BARRIER_XCD 2     /* two-level: members -> XCD leader, leaders <-> leaders */
