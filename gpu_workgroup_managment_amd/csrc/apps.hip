@@ -47,40 +47,60 @@ __global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDe
   standalone_retire(c);
 }
 
-__global__ void __launch_bounds__(1024) k_color(Ctl c, GraphDev g, int32_t* color, int* iters_out) {
+// wl: two lists of n entries; a workgroup uses [lo, hi) of each for the open
+// vertices of its block (worklist rounds). The open count of round r is
+// s_wl[r % 3]: reset two rounds after its last read.
+__global__ void __launch_bounds__(1024) k_color(Ctl c, GraphDev g, int32_t* color, int32_t* wl, int* iters_out) {
+  __shared__ int s_wl[3];
   Team t;
   if (!standalone_begin(c, t)) return;
   int lo, hi;
   block_range(g.n, t.id, t.n, &lo, &hi);
+  if (threadIdx.x == 0) s_wl[0] = s_wl[1] = s_wl[2] = 0;
+  __syncthreads();
   int* flags = c.lines;
-  int round = 1;
+  int round = 1, cnt = hi - lo;
+  const int32_t* lin = nullptr;
   for (;; round++) {
-    const bool any = color_round<true>(g, color, round, lo, hi);
+    if (threadIdx.x == 0) s_wl[(round + 1) % 3] = 0;
+    int32_t* lout = (round & 1) ? wl : wl + g.n;
+    const bool any = color_round_wl(g, color, round, lo, cnt, lin, lout, &s_wl[round % 3]);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
     if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
     xcd_barrier(c, t);
     if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
+    cnt = s_wl[round % 3];
+    lin = lout;
   }
   clock_end(c, t.id);
   if (t.id == 0 && threadIdx.x == 0) *iters_out = round;
   standalone_retire(c);
 }
 
-__global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, int32_t* s_out, int* iters_out) {
+__global__ void __launch_bounds__(1024) k_mis(Ctl c, GraphDev g, int32_t* st, int32_t* s_out, int32_t* wl,
+                                             int* iters_out) {
+  __shared__ int s_wl[3];
   Team t;
   if (!standalone_begin(c, t)) return;
   int lo, hi;
   block_range(g.n, t.id, t.n, &lo, &hi);
+  if (threadIdx.x == 0) s_wl[0] = s_wl[1] = s_wl[2] = 0;
+  __syncthreads();
   int* flags = c.lines;
-  int round = 1;
+  int round = 1, cnt = hi - lo;
+  const int32_t* lin = nullptr;
   for (;; round++) {
-    const bool any = mis_phase_a<true>(g, st, s_out, round, lo, hi);
+    if (threadIdx.x == 0) s_wl[(round + 1) % 3] = 0;
+    int32_t* lout = (round & 1) ? wl : wl + g.n;
+    const bool any = mis_phase_a_wl(g, st, s_out, round, lo, cnt, lin);
     if (__syncthreads_or(any) && threadIdx.x == 0) st_relaxed(&flags[(round % 3) * kLine], 1);
     if (t.id == 0 && threadIdx.x == 0) st_relaxed(&flags[((round + 1) % 3) * kLine], 0);
     xcd_barrier(c, t);
     if (ld_relaxed(&flags[(round % 3) * kLine]) == 0) break;
-    mis_phase_b<true>(g, st, round, lo, hi);
+    mis_phase_b_wl(g, st, round, lo, cnt, lin, lout, &s_wl[round % 3]);
     xcd_barrier(c, t);
+    cnt = s_wl[round % 3];
+    lin = lout;
   }
   clock_end(c, t.id);
   if (t.id == 0 && threadIdx.x == 0) *iters_out = round;
@@ -302,11 +322,11 @@ int wgm_app_probe(wgm_runtime* rt, int kernel, int grid, int wg_size) {
       return WGM_OK;
     case WGM_K_COLOR:
       hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (int32_t*)nullptr,
-                         (int*)nullptr);
+                         (int32_t*)nullptr, (int*)nullptr);
       return WGM_OK;
     case WGM_K_MIS:
       hipLaunchKernelGGL(k_mis, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (int32_t*)nullptr,
-                         (int32_t*)nullptr, (int*)nullptr);
+                         (int32_t*)nullptr, (int32_t*)nullptr, (int*)nullptr);
       return WGM_OK;
     case WGM_K_PAGERANK:
       hipLaunchKernelGGL(k_pagerank, dim3(grid), dim3(wg_size), 0, rt->stream, rt->ctl, GraphDev{}, (float*)nullptr,
@@ -358,7 +378,7 @@ int wgm_graph_destroy(wgm_graph* g) {
   wgm_pannotia_release(g);
   void* ptrs[] = {g->row_ptr, g->col, g->wt, g->prio, g->dist, g->stamp, g->q0, g->q1, g->counts,
                   g->i32a, g->i32b, g->f0, g->f1, g->f2, g->iters, g->tq, g->tqa, g->claims, g->far,
-                  g->in_far, g->fcnt, g->hv, g->hdu, g->hclaim, g->hcnt};
+                  g->in_far, g->fcnt, g->hv, g->hdu, g->hclaim, g->hcnt, g->wl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete g;
@@ -485,12 +505,14 @@ int wgm_color(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* colo
   if (!rt || !g || g->rt != rt) return fail(rt, WGM_ERR_INVALID, "bad runtime/graph");
   if (!g->prio) return fail(rt, WGM_ERR_INVALID, "colouring needs vertex priorities");
   int rc = dev_alloc(rt, &g->i32a, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->wl, 2 * (size_t)g->n);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, -1);
   int grid = 0, wg = 0;
   rc = launch_prologue(rt, WGM_K_COLOR, lc, &grid, &wg);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->iters);
+  hipLaunchKernelGGL(k_color, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->wl,
+                     g->iters);
   rc = launch_epilogue(rt, lc, g, st);
   if (rc) return rc;
   if (color_out) HIPCHK(rt, hipMemcpy(color_out, g->i32a, sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost));
@@ -502,13 +524,14 @@ int wgm_mis(wgm_runtime* rt, wgm_graph* g, const wgm_launch* lc, int32_t* s_out,
   if (!g->prio) return fail(rt, WGM_ERR_INVALID, "MIS needs vertex priorities");
   int rc = dev_alloc(rt, &g->i32a, g->n);
   if (!rc) rc = dev_alloc(rt, &g->i32b, g->n);
+  if (!rc) rc = dev_alloc(rt, &g->wl, 2 * (size_t)g->n);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32a, g->n, -1);
   hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, rt->stream, g->i32b, g->n, 0);
   int grid = 0, wg = 0;
   rc = launch_prologue(rt, WGM_K_MIS, lc, &grid, &wg);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_mis, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->i32b,
+  hipLaunchKernelGGL(k_mis, dim3(grid), dim3(wg), 0, rt->stream, rt->ctl, wgm_graph_view(g), g->i32a, g->i32b, g->wl,
                      g->iters);
   rc = launch_epilogue(rt, lc, g, st);
   if (rc) return rc;

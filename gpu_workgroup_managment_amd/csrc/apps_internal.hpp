@@ -24,6 +24,7 @@ struct wgm_graph {
   float* f1 = nullptr;
   float* f2 = nullptr;
   int* iters = nullptr;
+  int32_t* wl = nullptr;  // colour / MIS worklists: two lists of n entries
   // standalone BFS / SSSP (one team of every participant): two queues of
   // tcap entries with distance tags (SSSP), claim counters, near-far queues
   // and flags

@@ -631,6 +631,202 @@ __device__ __forceinline__ void mis_phase_b(const GraphDev& g, int32_t* st, int 
   }
 }
 
+// ------------------------------------------------------- worklist rounds
+// The standalone colour and MIS kernels visit only the vertices still open:
+// a workgroup keeps the open vertices of its block in its share ([lo, ...))
+// of one of two lists, visits that list in the next round and writes the
+// vertices that stay open to the other. A vertex's decision depends only on
+// its neighbours' states, read as in the topology-driven rounds, and only its
+// own workgroup writes it, so the colours, sets and round counts are the same.
+// On the 2048^2 grid colour's 16 rounds visit 13.9 M vertices instead of
+// 67.1 M. The first round visits the whole block (no list).
+__device__ __forceinline__ void wl_push(bool stay, int v, int32_t* lout, int lo, int* cnt) {
+  const unsigned long long mask = __ballot(stay);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(cnt, __popcll(mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (stay) lout[lo + base + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+  }
+}
+
+// The i-th open vertex of the block: list entry, or lo + i in the first round.
+__device__ __forceinline__ int wl_vertex(const int32_t* lin, int lo, int i) { return lin ? lin[lo + i] : lo + i; }
+
+__device__ __forceinline__ bool color_round_wl(const GraphDev& g, int32_t* color, int round, int lo, int cnt,
+                                               const int32_t* lin, int32_t* lout, int* ocnt) {
+  bool any = false;
+  const int bd = blockDim.x;
+  for (int base = (int)threadIdx.x; base < cnt; base += kTopoGroup * bd) {
+    int v[kTopoGroup], beg[kTopoGroup], deg[kTopoGroup];
+    uint32_t pv[kTopoGroup];
+    bool open[kTopoGroup];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      open[k] = base + k * bd < cnt;
+      v[k] = open[k] ? wl_vertex(lin, lo, base + k * bd) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
+      const int end = open[k] ? g.row_ptr[v[k] + 1] : 0;
+      pv[k] = open[k] ? g.prio[v[k]] : 0u;
+      deg[k] = end - beg[k];
+      if (deg[k] == 1) deg[k] = 0;  // a vertex with exactly one neighbour ignores it
+    }
+    int u[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
+    int cu[kTopoGroup][kTopoEdges];
+    uint32_t pu[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) {
+        cu[k][j] = u[k][j] >= 0 ? color[u[k][j]] : 0;
+        pu[k][j] = u[k][j] >= 0 ? g.prio[u[k][j]] : 0u;
+      }
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      bool coloured = false;
+      if (open[k]) {
+        long long mx = -1;
+#pragma unroll
+        for (int j = 0; j < kTopoEdges; j++) {
+          if (u[k][j] >= 0 && (cu[k][j] == -1 || cu[k][j] == round)) {
+            any = true;
+            mx = (long long)pu[k][j] > mx ? (long long)pu[k][j] : mx;
+          }
+        }
+        for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k]; e++) {
+          const int w = g.col[e];
+          const int cw = color[w];
+          if (cw == -1 || cw == round) {
+            any = true;
+            const long long pw = (long long)g.prio[w];
+            mx = pw > mx ? pw : mx;
+          }
+        }
+        coloured = (long long)pv[k] > mx;
+        if (coloured) color[v[k]] = round;
+      }
+      wl_push(open[k] && !coloured, v[k], lout, lo, ocnt);
+    }
+  }
+  return any;
+}
+
+// MIS phase A over the open list: an open vertex joins if its priority is
+// below every neighbour undecided at the round start (state -1 or 2r).
+__device__ __forceinline__ bool mis_phase_a_wl(const GraphDev& g, int32_t* st, int32_t* s_out, int round, int lo,
+                                               int cnt, const int32_t* lin) {
+  bool any = false;
+  const int joined = 2 * round;
+  const int bd = blockDim.x;
+  for (int base = (int)threadIdx.x; base < cnt; base += kTopoGroup * bd) {
+    int v[kTopoGroup], beg[kTopoGroup], deg[kTopoGroup];
+    uint32_t pv[kTopoGroup];
+    bool open[kTopoGroup];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      open[k] = base + k * bd < cnt;
+      v[k] = open[k] ? wl_vertex(lin, lo, base + k * bd) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
+      deg[k] = (open[k] ? g.row_ptr[v[k] + 1] : 0) - beg[k];
+      pv[k] = open[k] ? g.prio[v[k]] : 0u;
+    }
+    int u[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
+    int su[kTopoGroup][kTopoEdges];
+    uint32_t pu[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) {
+        su[k][j] = u[k][j] >= 0 ? st[u[k][j]] : 0;
+        pu[k][j] = u[k][j] >= 0 ? g.prio[u[k][j]] : 0u;
+      }
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      if (!open[k]) continue;
+      any = true;
+      long long mn = 0x100000000LL;  // above every uint32 priority
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++)
+        if (u[k][j] >= 0 && (su[k][j] == -1 || su[k][j] == joined))
+          mn = (long long)pu[k][j] < mn ? (long long)pu[k][j] : mn;
+      for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k]; e++) {
+        const int w = g.col[e];
+        const int sw = st[w];
+        if (sw == -1 || sw == joined) {
+          const long long pw = (long long)g.prio[w];
+          mn = pw < mn ? pw : mn;
+        }
+      }
+      if ((long long)pv[k] < mn) {
+        st[v[k]] = joined;
+        s_out[v[k]] = 2;
+      }
+    }
+  }
+  return any;
+}
+
+// MIS phase B over the same list: a vertex that did not join and has a
+// neighbour that joined this round is removed; the rest stay open.
+__device__ __forceinline__ void mis_phase_b_wl(const GraphDev& g, int32_t* st, int round, int lo, int cnt,
+                                               const int32_t* lin, int32_t* lout, int* ocnt) {
+  const int joined = 2 * round;
+  const int bd = blockDim.x;
+  for (int base = (int)threadIdx.x; base < cnt; base += kTopoGroup * bd) {
+    int v[kTopoGroup], sv[kTopoGroup], beg[kTopoGroup], deg[kTopoGroup];
+    bool open[kTopoGroup];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      const bool in = base + k * bd < cnt;
+      v[k] = in ? wl_vertex(lin, lo, base + k * bd) : 0;
+      sv[k] = in ? st[v[k]] : 0;
+      open[k] = in && sv[k] == -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
+      deg[k] = (open[k] ? g.row_ptr[v[k] + 1] : 0) - beg[k];
+    }
+    int u[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
+    int su[kTopoGroup][kTopoEdges];
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++)
+#pragma unroll
+      for (int j = 0; j < kTopoEdges; j++) su[k][j] = u[k][j] >= 0 ? st[u[k][j]] : 0;
+#pragma unroll
+    for (int k = 0; k < kTopoGroup; k++) {
+      bool hit = false;
+      if (open[k]) {
+#pragma unroll
+        for (int j = 0; j < kTopoEdges; j++) hit = hit || (u[k][j] >= 0 && su[k][j] == joined);
+        for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k] && !hit; e++) hit = st[g.col[e]] == joined;
+        if (hit) st[v[k]] = joined + 1;
+      }
+      wl_push(open[k] && !hit, v[k], lout, lo, ocnt);
+    }
+  }
+}
+
 // ------------------------------------------------------------ PageRank
 // Pull iteration r'(v) = base + d * sum_{u in N(v)} r(u)/deg(u), FP32, the sum
 // taken in CSR order by one thread, so the result is bitwise reproducible.
