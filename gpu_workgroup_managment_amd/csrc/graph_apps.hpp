@@ -390,97 +390,37 @@ __device__ __forceinline__ void frontier_heavy(const GraphDev& g, const Frontier
 // (`start != end - 1`, src/pannotia/graph_apps/color/device/color.cl:46) is kept:
 // a vertex with exactly one neighbour ignores it. The stop flag of round c is
 // line c % 3 of `flags`.
-// GROUPED (the standalone kernels): vertices go kTopoGroup to a thread per
-// pass (v, v + bd, v + 2 bd, ...: each load coalesced across the wave), and
-// the work of the group is issued level by level -- their colours, then rows
-// and priorities, then the first kTopoEdges edges of each, then those
-// neighbours' colours and priorities -- so a thread waits for four memory
-// round trips per group instead of a dependent chain per vertex and per edge.
-// Edges past the first kTopoEdges of a vertex are walked one by one
+// The standalone kernels (the worklist rounds below) take vertices
+// kTopoGroup to a thread per pass (i, i + bd, i + 2 bd, ...: each load
+// coalesced across the wave) and issue the group's work level by level --
+// their rows and priorities, then the first kTopoEdges edges of each, then
+// those neighbours' states and priorities -- so a thread waits for a few
+// memory round trips per group instead of a dependent chain per vertex and
+// per edge. Edges past the first kTopoEdges of a vertex are walked one by one
 // afterwards. The registers this takes (~90 VGPRs) would spill in the
-// mega-kernel, whose resizable tasks keep one vertex at a time.
+// mega-kernel, whose resizable tasks keep one vertex at a time (color_round,
+// mis_phase_a / _b).
 constexpr int kTopoGroup = 4;
 constexpr int kTopoEdges = 4;
 
-template <bool GROUPED = false>
 __device__ __forceinline__ bool color_round(const GraphDev& g, int32_t* color, int round, int lo, int hi) {
   bool any = false;
-  if (!GROUPED) {
-    for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-      if (color[v] != -1) continue;
-      const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
-      long long mx = -1;
-      if (end - beg != 1) {
-        for (int e = beg; e < end; e++) {
-          const int u = g.col[e];
-          const int cu = color[u];
-          if (cu == -1 || cu == round) {
-            any = true;
-            const long long pu = (long long)g.prio[u];
-            mx = pu > mx ? pu : mx;
-          }
-        }
-      }
-      if ((long long)g.prio[v] > mx) color[v] = round;
-    }
-    return any;
-  }
-  const int bd = blockDim.x;
-  for (int base = lo + (int)threadIdx.x; base < hi; base += kTopoGroup * bd) {
-    int v[kTopoGroup], cv[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      v[k] = base + k * bd;
-      cv[k] = v[k] < hi ? color[v[k]] : 0;
-    }
-    int beg[kTopoGroup], deg[kTopoGroup];
-    uint32_t pv[kTopoGroup];
-    bool open[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      open[k] = v[k] < hi && cv[k] == -1;
-      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
-      const int end = open[k] ? g.row_ptr[v[k] + 1] : 0;
-      pv[k] = open[k] ? g.prio[v[k]] : 0u;
-      deg[k] = end - beg[k];
-      if (deg[k] == 1) deg[k] = 0;  // a vertex with exactly one neighbour ignores it
-    }
-    int u[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
-    int cu[kTopoGroup][kTopoEdges];
-    uint32_t pu[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) {
-        cu[k][j] = u[k][j] >= 0 ? color[u[k][j]] : 0;
-        pu[k][j] = u[k][j] >= 0 ? g.prio[u[k][j]] : 0u;
-      }
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      if (!open[k]) continue;
-      long long mx = -1;
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) {
-        if (u[k][j] >= 0 && (cu[k][j] == -1 || cu[k][j] == round)) {
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (color[v] != -1) continue;
+    const int beg = g.row_ptr[v], end = g.row_ptr[v + 1];
+    long long mx = -1;
+    if (end - beg != 1) {
+      for (int e = beg; e < end; e++) {
+        const int u = g.col[e];
+        const int cu = color[u];
+        if (cu == -1 || cu == round) {
           any = true;
-          mx = (long long)pu[k][j] > mx ? (long long)pu[k][j] : mx;
+          const long long pu = (long long)g.prio[u];
+          mx = pu > mx ? pu : mx;
         }
       }
-      for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k]; e++) {
-        const int w = g.col[e];
-        const int cw = color[w];
-        if (cw == -1 || cw == round) {
-          any = true;
-          const long long pw = (long long)g.prio[w];
-          mx = pw > mx ? pw : mx;
-        }
-      }
-      if ((long long)pv[k] > mx) color[v[k]] = round;
     }
+    if ((long long)g.prio[v] > mx) color[v] = round;
   }
   return any;
 }
@@ -493,140 +433,41 @@ __device__ __forceinline__ bool color_round(const GraphDev& g, int32_t* color, i
 // undecided vertex with a neighbour that joined this round is removed.
 // Two barriers per round instead of three; the sets are those of
 // src/pannotia/graph_apps/mis/device/mis.cl:21-124.
-// GROUPED: both phases take vertices kTopoGroup to a thread per pass with
-// their loads issued level by level, as color_round does.
-template <bool GROUPED = false>
+// The standalone kernel's forms (mis_phase_a_wl / _b_wl) group their loads
+// as the colour rounds do.
 __device__ __forceinline__ bool mis_phase_a(const GraphDev& g, int32_t* st, int32_t* s_out, int round,
                                             int lo, int hi) {
   bool any = false;
   const int joined = 2 * round;
-  if (!GROUPED) {
-    for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-      if (st[v] != -1) continue;
-      any = true;
-      long long mn = 0x100000000LL;  // above every uint32 priority
-      for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
-        const int u = g.col[e];
-        const int su = st[u];
-        if (su == -1 || su == joined) {
-          const long long pu = (long long)g.prio[u];
-          mn = pu < mn ? pu : mn;
-        }
-      }
-      if ((long long)g.prio[v] < mn) {
-        st[v] = joined;
-        s_out[v] = 2;
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (st[v] != -1) continue;
+    any = true;
+    long long mn = 0x100000000LL;  // above every uint32 priority
+    for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
+      const int u = g.col[e];
+      const int su = st[u];
+      if (su == -1 || su == joined) {
+        const long long pu = (long long)g.prio[u];
+        mn = pu < mn ? pu : mn;
       }
     }
-    return any;
-  }
-  const int bd = blockDim.x;
-  for (int base = lo + (int)threadIdx.x; base < hi; base += kTopoGroup * bd) {
-    int v[kTopoGroup], sv[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      v[k] = base + k * bd;
-      sv[k] = v[k] < hi ? st[v[k]] : 0;
-    }
-    int beg[kTopoGroup], deg[kTopoGroup];
-    uint32_t pv[kTopoGroup];
-    bool open[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      open[k] = v[k] < hi && sv[k] == -1;
-      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
-      deg[k] = (open[k] ? g.row_ptr[v[k] + 1] : 0) - beg[k];
-      pv[k] = open[k] ? g.prio[v[k]] : 0u;
-    }
-    int u[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
-    int su[kTopoGroup][kTopoEdges];
-    uint32_t pu[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) {
-        su[k][j] = u[k][j] >= 0 ? st[u[k][j]] : 0;
-        pu[k][j] = u[k][j] >= 0 ? g.prio[u[k][j]] : 0u;
-      }
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      if (!open[k]) continue;
-      any = true;
-      long long mn = 0x100000000LL;  // above every uint32 priority
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++)
-        if (u[k][j] >= 0 && (su[k][j] == -1 || su[k][j] == joined))
-          mn = (long long)pu[k][j] < mn ? (long long)pu[k][j] : mn;
-      for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k]; e++) {
-        const int w = g.col[e];
-        const int sw = st[w];
-        if (sw == -1 || sw == joined) {
-          const long long pw = (long long)g.prio[w];
-          mn = pw < mn ? pw : mn;
-        }
-      }
-      if ((long long)pv[k] < mn) {
-        st[v[k]] = joined;
-        s_out[v[k]] = 2;
-      }
+    if ((long long)g.prio[v] < mn) {
+      st[v] = joined;
+      s_out[v] = 2;
     }
   }
   return any;
 }
 
-template <bool GROUPED = false>
 __device__ __forceinline__ void mis_phase_b(const GraphDev& g, int32_t* st, int round, int lo, int hi) {
   const int joined = 2 * round;
-  if (!GROUPED) {
-    for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
-      if (st[v] != -1) continue;
-      for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
-        if (st[g.col[e]] == joined) {
-          st[v] = joined + 1;
-          break;
-        }
+  for (int v = lo + (int)threadIdx.x; v < hi; v += blockDim.x) {
+    if (st[v] != -1) continue;
+    for (int e = g.row_ptr[v]; e < g.row_ptr[v + 1]; e++) {
+      if (st[g.col[e]] == joined) {
+        st[v] = joined + 1;
+        break;
       }
-    }
-    return;
-  }
-  const int bd = blockDim.x;
-  for (int base = lo + (int)threadIdx.x; base < hi; base += kTopoGroup * bd) {
-    int v[kTopoGroup], sv[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      v[k] = base + k * bd;
-      sv[k] = v[k] < hi ? st[v[k]] : 0;
-    }
-    int beg[kTopoGroup], deg[kTopoGroup];
-    bool open[kTopoGroup];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      open[k] = v[k] < hi && sv[k] == -1;
-      beg[k] = open[k] ? g.row_ptr[v[k]] : 0;
-      deg[k] = (open[k] ? g.row_ptr[v[k] + 1] : 0) - beg[k];
-    }
-    int u[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) u[k][j] = j < deg[k] ? g.col[beg[k] + j] : -1;
-    int su[kTopoGroup][kTopoEdges];
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++)
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) su[k][j] = u[k][j] >= 0 ? st[u[k][j]] : 0;
-#pragma unroll
-    for (int k = 0; k < kTopoGroup; k++) {
-      if (!open[k]) continue;
-      bool hit = false;
-#pragma unroll
-      for (int j = 0; j < kTopoEdges; j++) hit = hit || (u[k][j] >= 0 && su[k][j] == joined);
-      for (int e = beg[k] + kTopoEdges; e < beg[k] + deg[k] && !hit; e++) hit = st[g.col[e]] == joined;
-      if (hit) st[v[k]] = joined + 1;
     }
   }
 }
