@@ -182,10 +182,163 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
   }
 }
 
+// relax_push for U edges per lane: edges idx[0..U) of sources at distances
+// du[]. Each step's loads or atomics are issued for every slot before any
+// result is used (the results pass through opaque() after the issuing loop),
+// so a lane waits for one round trip per step instead of U; the wave then
+// pushes all U slots' next-level entries with one counter atomic and its far
+// entries with one more. Called by every lane of the wave; `active` marks the
+// slots with an edge.
+template <int U, bool SSSP, bool LOCAL, bool COH, bool PAIRS>
+__device__ __forceinline__ void relax_push_n(const GraphDev& g, const FrontierDev& s, const bool* active,
+                                             const int* idx, const uint32_t* du, int32_t* out_q, uint32_t* out_qa,
+                                             int* out_cnt, uint32_t next, uint32_t thr, int32_t* far_q, int* far_cnt,
+                                             uint32_t* in_far, lds_u32* bits = nullptr) {
+  constexpr int scope = LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
+  const bool packed = SSSP && g.pw;
+  uint32_t wd[U], wt[U];
+#pragma unroll
+  for (int x = 0; x < U; x++) {
+    wd[x] = 0u;
+    wt[x] = 0u;
+    if (active[x]) {
+      if (packed) {
+        wd[x] = g.pw[idx[x]];
+      } else {
+        wd[x] = (uint32_t)g.col[idx[x]];
+        if (SSSP) wt[x] = (uint32_t)g.wt[idx[x]];
+      }
+    }
+  }
+  int u[U];
+  uint32_t alt[U], cur[U];
+#pragma unroll
+  for (int x = 0; x < U; x++) {
+    opaque(wd[x]);
+    opaque(wt[x]);
+    u[x] = packed ? (int)(wd[x] >> 8) : (int)wd[x];
+    alt[x] = SSSP ? du[x] + (packed ? (wd[x] & 255u) : wt[x]) : next;
+    // the target's distance filters the atomic (a stale value is never
+    // below the current one); BFS with visited bits claims in one LDS step
+    cur[x] = 0u;
+    if (active[x]) {
+      if (!SSSP && LOCAL && bits) {
+        const uint32_t bit = 1u << (u[x] & 31);
+        cur[x] = (__hip_atomic_fetch_or(&bits[u[x] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit)
+                     ? 0u : kInf;
+      } else {
+        cur[x] = s.dist[u[x]];
+      }
+    }
+  }
+  uint32_t old[U];  // the target's distance before this slot's update (0: no update)
+#pragma unroll
+  for (int x = 0; x < U; x++) {
+    opaque(cur[x]);
+    old[x] = 0u;
+    if (!SSSP && LOCAL && bits) {
+      old[x] = cur[x];
+    } else if (!SSSP) {
+      if (active[x] && cur[x] == kInf) {
+        uint32_t expect = kInf;  // left at kInf by a successful exchange
+        __hip_atomic_compare_exchange_strong(&s.dist[u[x]], &expect, next, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             scope);
+        old[x] = expect;
+      }
+    } else if (active[x] && alt[x] < cur[x]) {
+      old[x] = __hip_atomic_fetch_min(&s.dist[u[x]], alt[x], __ATOMIC_RELAXED, scope);
+    }
+  }
+  bool near[U], farc[U];
+  uint32_t res[U];  // stamp before the update (near), or the far flag's previous state
+#pragma unroll
+  for (int x = 0; x < U; x++) {
+    opaque(old[x]);
+    const bool better = SSSP ? alt[x] < old[x] : old[x] == kInf;
+    near[x] = better && (!SSSP || alt[x] < thr);
+    farc[x] = SSSP && better && alt[x] >= thr;
+    res[x] = 0u;
+    if (!SSSP) {
+      if (LOCAL && bits && near[x]) s.dist[u[x]] = next;
+    } else if (near[x]) {
+      if (!PAIRS) res[x] = __hip_atomic_fetch_max(&s.stamp[u[x]], next, __ATOMIC_RELAXED, scope);
+    } else if (farc[x]) {
+      const uint32_t bit = 1u << (u[x] & 31);
+      if (LOCAL && bits)
+        res[x] = __hip_atomic_fetch_or(&bits[u[x] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit;
+      else
+        res[x] = __hip_atomic_exchange(&in_far[u[x]], 1u, __ATOMIC_RELAXED, scope);
+    }
+  }
+  bool want[U], far_want[U];
+  unsigned long long mask[U], fmask[U];
+  int cnt = 0, fcnt = 0;
+#pragma unroll
+  for (int x = 0; x < U; x++) {
+    opaque(res[x]);
+    want[x] = near[x] && (!SSSP || PAIRS || res[x] < next);
+    far_want[x] = farc[x] && res[x] == 0u;
+    mask[x] = __ballot(want[x]);
+    fmask[x] = __ballot(far_want[x]);
+    cnt += __popcll(mask[x]);
+    fcnt += __popcll(fmask[x]);
+  }
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int leader = __ffsll((long long)__ballot(1)) - 1;  // the first active lane
+  if (cnt) {
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(out_cnt, cnt, __ATOMIC_RELAXED, scope);
+    base = __builtin_amdgcn_readlane(base, leader);  // leader is wave-uniform: no LDS round trip
+#pragma unroll
+    for (int x = 0; x < U; x++) {
+      if (want[x]) {
+        const int pos = base + __popcll(mask[x] & below);
+        if (COH) {
+          __hip_atomic_store(&out_q[pos], u[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (PAIRS) __hip_atomic_store(&out_qa[pos], alt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          out_q[pos] = u[x];
+          if (PAIRS) out_qa[pos] = alt[x];
+        }
+      }
+      base += __popcll(mask[x]);
+    }
+  }
+  if (SSSP && far_q && fcnt) {
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(far_cnt, fcnt, __ATOMIC_RELAXED, scope);
+    base = __builtin_amdgcn_readlane(base, leader);
+#pragma unroll
+    for (int x = 0; x < U; x++) {
+      if (far_want[x]) {
+        int32_t* slot = &far_q[base + __popcll(fmask[x] & below)];
+        if (COH) __hip_atomic_store(slot, u[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *slot = u[x];
+      }
+      base += __popcll(fmask[x]);
+    }
+  }
+}
+
+// Edges each thread takes per pass of frontier_batch<..., U> in the
+// standalone SSSP kernel: their loads and atomics are in flight together
+// (relax_push_n), so a pass over a batch's edges costs about one chain of
+// round trips per kEdgeSlots edges per thread. RMAT scale 20 (ef 16) push
+// SSSP, 64 x 1024 workgroups: 3.99 ms at one slot, 3.52 at two, 3.29 at
+// three, 3.20 at four. BFS keeps one: its levels are short (ER 2^16: 0.099 ms
+// at one slot, 0.110 at four). The batched, serving and merged kernels keep
+// one too: their kernels' other paths are scheduled for a register budget
+// that the extra slots' state would exceed (the batched kernel's
+// single-workgroup grid SSSP ran 4% slower alone, 9% inside the batch).
+constexpr int kEdgeSlots = 4;
+
 // Relax the edges of frontier entries in_q[a, b) (b - a <= blockDim.x).
 // The workgroup scans the entries' degrees and spreads their edges evenly
-// over its threads, each finding its source entry by binary search in LDS,
-// so a high-degree vertex does not serialise one thread. BFS claims a vertex
+// over its threads, U per thread per pass (edges e, e + bd, ...), each
+// finding its source entry by binary search in LDS (with U > 1 the slots'
+// searches step together), so a high-degree vertex does not serialise one
+// thread. BFS claims a vertex
 // with one CAS (INF -> level+1) and pushes it exactly once; SSSP lowers dist
 // with atomic min and pushes a vertex at most once per level through
 // `stamp`. Pushes are wave-aggregated: one counter atomic per wave.
@@ -203,7 +356,7 @@ __device__ __forceinline__ void relax_push(const GraphDev& g, const FrontierDev&
 // once while its in_far flag is set; the caller processes it in a later
 // band. Final distances are the same (they are unique); the work is closer
 // to Dijkstra's order.
-template <bool SSSP, bool LOCAL, bool COH = false, bool PAIRS = false>
+template <bool SSSP, bool LOCAL, bool COH = false, bool PAIRS = false, int U = 1>
 __device__ __forceinline__ void frontier_batch(const GraphDev& g, const FrontierDev& s, const int32_t* in_q, int a,
                                                int b, int32_t* out_q, int* out_cnt, uint32_t next, int* lds,
                                                uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
@@ -258,21 +411,57 @@ __device__ __forceinline__ void frontier_batch(const GraphDev& g, const Frontier
   s_beg[tid] = beg - excl;
   if (SSSP) s_du[tid] = du;
   __syncthreads();
-  for (int eb = 0; eb < total; eb += bd) {
-    const int e = eb + tid;
-    int idx = 0;
-    uint32_t du_e = 0;
-    if (e < total) {
-      int lo = 0, hi = bd - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s_end[mid] > e) hi = mid; else lo = mid + 1;
+  if constexpr (U == 1) {
+    for (int eb = 0; eb < total; eb += bd) {
+      const int e = eb + tid;
+      int idx = 0;
+      uint32_t du_e = 0;
+      if (e < total) {
+        int lo = 0, hi = bd - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_end[mid] > e) hi = mid; else lo = mid + 1;
+        }
+        idx = s_beg[lo] + e;
+        du_e = SSSP ? s_du[lo] : 0u;
       }
-      idx = s_beg[lo] + e;
-      du_e = SSSP ? s_du[lo] : 0u;
+      relax_push<SSSP, LOCAL, COH, PAIRS>(g, s, e < total, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q,
+                                          far_cnt, in_far, bits);
     }
-    relax_push<SSSP, LOCAL, COH, PAIRS>(g, s, e < total, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q, far_cnt,
-                                        in_far, bits);
+  } else {
+    // bisection steps that narrow bd candidates to one: ceil(log2 bd)
+    const int steps = 32 - __clz(bd - 1);
+    for (int eb = 0; eb < total; eb += U * bd) {
+      int e[U], lo[U], hi[U], idx[U];
+      bool act[U];
+      uint32_t du_e[U];
+#pragma unroll
+      for (int x = 0; x < U; x++) {
+        e[x] = eb + x * bd + tid;
+        act[x] = e[x] < total;
+        lo[x] = 0;
+        hi[x] = act[x] ? bd - 1 : 0;
+      }
+      // the first entry whose end is past e: the slots' LDS reads of one step
+      // are issued together (a settled slot re-reads its own entry)
+      for (int k = 0; k < steps; k++) {
+#pragma unroll
+        for (int x = 0; x < U; x++) {
+          const int mid = (lo[x] + hi[x]) >> 1;
+          const bool past = s_end[mid] > e[x];
+          const bool open = lo[x] < hi[x];
+          hi[x] = open && past ? mid : hi[x];
+          lo[x] = open && !past ? mid + 1 : lo[x];
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < U; x++) {
+        idx[x] = act[x] ? s_beg[lo[x]] + e[x] : 0;
+        du_e[x] = SSSP && act[x] ? s_du[lo[x]] : 0u;
+      }
+      relax_push_n<U, SSSP, LOCAL, COH, PAIRS>(g, s, act, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q,
+                                               far_cnt, in_far, bits);
+    }
   }
   __syncthreads();
 }
@@ -303,7 +492,7 @@ struct NearFar {
 // idle while those walk every edge. Members may claim different sizes (a
 // claim takes [c0, c0 + its size) from the counter), so the claimed ranges
 // still tile the level.
-template <bool SSSP, bool COH = false, bool PAIRS = false>
+template <bool SSSP, bool COH = false, bool PAIRS = false, int U = 1>
 __device__ __forceinline__ void frontier_level(const GraphDev& g, const FrontierDev& s, int level,
                                                int rank, int nteam, int* lds, uint32_t thr = 0xffffffffu,
                                                int32_t* far_q = nullptr, int* far_cnt = nullptr,
@@ -329,14 +518,14 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
       if (c0 >= in_n) break;
       const int c1 = min(c0 + chunk, in_n);
       for (int a = c0; a < c1; a += bd)
-        frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr,
-                                                far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
+        frontier_batch<SSSP, false, COH, PAIRS, U>(g, s, in_q, a, min(a + bd, c1), out_q, out_cnt, next, lds, thr,
+                                                   far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
     }
     if (rank == 0 && tid == 0) st_relaxed(&s.claims[((level + 2) % 3) * kLine], 0);
   } else {
     for (int a = rank * bd; a < in_n; a += nteam * bd)
-      frontier_batch<SSSP, false, COH, PAIRS>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr,
-                                              far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
+      frontier_batch<SSSP, false, COH, PAIRS, U>(g, s, in_q, a, min(a + bd, in_n), out_q, out_cnt, next, lds, thr,
+                                                 far_q, far_cnt, in_far, in_qa, out_qa, heavy_cnt);
   }
   if (rank == 0 && tid == 0) {
     st_relaxed(&s.counts[((level + 2) % 3) * kLine], 0);

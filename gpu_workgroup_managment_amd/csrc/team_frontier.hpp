@@ -123,9 +123,10 @@ __device__ __forceinline__ void frontier_bottom_up(const GraphDev& g, const Fron
 // team's barrier) only has to drain, never to write back or invalidate an
 // L2. SSSP runs near-far bands of width nf.delta (0: frontier
 // Bellman-Ford), with the far counters and band minimum in device lines.
+// U: edges per thread per frontier pass (frontier_batch).
 // Reference: drelax2 (src/lonestar/graph_apps/bfs/device/bfs.cl:214-302,
 // src/lonestar/graph_apps/sssp/device/sssp.cl:262-318).
-template <bool SSSP, class Barrier>
+template <bool SSSP, int U = 1, class Barrier>
 __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int rank, int k,
                          int* lds, int* iters_out, Barrier&& bar) {
   const bool near_far = SSSP && nf.delta > 0;
@@ -163,8 +164,8 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
       if (!SSSP && g.rrow && in_n > g.n / kBottomUpDiv)
         frontier_bottom_up(g, s, level, rank, k);
       else
-        frontier_level<SSSP, true, SSSP>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr,
-                                         &nf.cnt[fi * kLine], nf.flag);
+        frontier_level<SSSP, true, SSSP, U>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr,
+                                            &nf.cnt[fi * kLine], nf.flag);
       bar();
       // heavy entries deferred by the pass (read after the barrier: every
       // member sees the same count and takes the same number of barriers)

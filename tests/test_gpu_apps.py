@@ -80,6 +80,19 @@ def test_sssp_rmat(rt, scale, wgs, mode):
     print(st)
 
 
+@pytest.mark.parametrize("wg", [64, 192, 1024])
+def test_sssp_push_workgroup_sizes(rt, wg):
+    """Push SSSP spreads a batch's edges several to a thread per pass, each
+    slot finding its source entry by a bisection of ceil(log2 wg) steps: exact
+    at a workgroup size that is not a power of two, at the smallest and at the
+    largest, on a skewed and on a low-degree graph."""
+    for g in (G.add_weights(G.rmat(12, 16, seed=5), 1, 255, 6), G.add_weights(G.grid(90, 70), 1, 255, 7)):
+        dist, st = rt.graph(g).sssp(0, wg_size=wg, num_wgs=0, mode=L.SSSP_PUSH)
+        ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
+        assert np.array_equal(dist, ref), (wg, g.n, _first_diff(dist, ref))
+        assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())
+
+
 @pytest.mark.parametrize("mode", [L.SSSP_AUTO, L.SSSP_PULL])
 def test_sssp_grid_unit_weights(rt, mode):
     side = 128
