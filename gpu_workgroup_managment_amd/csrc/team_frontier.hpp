@@ -73,7 +73,11 @@ __device__ __forceinline__ void team_barrier(int* cnt, int* gen, int k, int& epo
 // discovers is the top-down level's: the unvisited vertices with an in-edge
 // from the frontier. The level's counters are rotated as frontier_level
 // rotates them.
-constexpr int kBottomUpDiv = 16;  // bottom-up while the frontier holds more than n / 16 vertices
+// A level runs bottom-up while the frontier holds more than n / kBottomUpDiv
+// vertices, or more than the vertices not yet reached (the tail of a BFS,
+// where a few unvisited vertices scan their in-edges instead of the frontier
+// its out-edges).
+constexpr int kBottomUpDiv = 16;
 
 __device__ __forceinline__ void frontier_bottom_up(const GraphDev& g, const FrontierDev& s, int level, int rank,
                                                    int nteam) {
@@ -156,12 +160,14 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
   bar();
   const int bd = blockDim.x;
   int level = 0, fi = 0;
+  int seen = 0;  // BFS: vertices reached so far, this level's frontier included
   uint32_t thr = near_far ? nf.delta : 0xffffffffu;
   while (true) {
     int in_n;
     while ((in_n = ld_relaxed(&s.counts[(level % 3) * kLine])) != 0) {
-      // every member read the same count after the barrier: the same choice
-      if (!SSSP && g.rrow && in_n > g.n / kBottomUpDiv)
+      // every member read the same counts after the barriers: the same choice
+      seen += in_n;
+      if (!SSSP && g.rrow && (in_n > g.n / kBottomUpDiv || in_n > g.n - seen))
         frontier_bottom_up(g, s, level, rank, k);
       else
         frontier_level<SSSP, true, SSSP, U>(g, s, level, rank, k, lds, thr, near_far ? nf.far(fi) : nullptr,
