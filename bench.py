@@ -226,6 +226,17 @@ def serving_sweep(rt, graphs, queries, batch_digests, loads, duration_s, np_frac
     }
 
 
+def barrier_cases(coop=False):
+    """(variant, name, workgroup size) of the barrier round-trip measurements
+    at full occupancy. Every wait of these barriers has a spin budget (a
+    barrier that never completes ends the launch with an error); HIP's
+    cooperative grid sync has none, so it runs only on request. The two-level
+    barrier at 256-thread workgroups is left out: one run at that shape once
+    did not complete, with the cause not shown."""
+    variants = [(2, "xcd"), (0, "flags"), (1, "sense")] + ([(3, "hip_cooperative_grid_sync")] if coop else [])
+    return [(var, name, wg) for var, name in variants for wg in (1024, 256) if not (var == 2 and wg == 256)]
+
+
 def configs_leg(rt, reps=5, coop=False):
     """The other configured workloads on their standalone kernels, each on its
     full input and checked against the oracle, for the one JSON line: BFS on
@@ -327,26 +338,19 @@ def configs_leg(rt, reps=5, coop=False):
                                        "l1_vs_fp64": float(np.abs(res.astype(np.float64) - p64).sum()),
                                        "bitexact_vs": "FP32 oracle", "cpu_threads": 1})
     dg.close()
-    # 4. barrier round trip at full occupancy (os_freq): 10 x iterations barriers per launch. Every wait
-    # of these barriers has a spin budget (a barrier that never completes ends the run with an error).
-    # HIP's cooperative grid sync has none, so it runs only on request (--coop-barrier).
+    # 4. barrier round trip at full occupancy (os_freq): 10 x iterations barriers per launch, the
+    # cases of barrier_cases
     bar = {}
-    variants = [(2, "xcd"), (0, "flags"), (1, "sense")]
-    if coop:
-        variants.append((3, "hip_cooperative_grid_sync"))
-    for var, name in variants:
-        for wg in (1024, 256):
-            if var == 2 and wg == 256:
-                continue  # once did not complete at this shape (cause not shown): kept off
-            log(f"config barrier {name} wg {wg}")
-            try:
-                r = rt.barrier_bench(wg, 0, 200, var)
-            except RuntimeError as e:  # WgmError: a wait ran out of its spin budget; reported, not fatal
-                bar[f"{name}_wg{wg}"] = {"error": str(e)}
-                log(f"config barrier {name} wg {wg}: {e}")
-                continue
-            bar[f"{name}_wg{wg}"] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
-                                     "epoch_errors": r["epoch_errors"]}
+    for var, name, wg in barrier_cases(coop):
+        log(f"config barrier {name} wg {wg}")
+        try:
+            r = rt.barrier_bench(wg, 0, 200, var)
+        except RuntimeError as e:  # WgmError: a wait ran out of its spin budget; reported, not fatal
+            bar[f"{name}_wg{wg}"] = {"error": str(e)}
+            log(f"config barrier {name} wg {wg}: {e}")
+            continue
+        bar[f"{name}_wg{wg}"] = {"participating": r["participating"], "ns_per_barrier": round(r["ns_per_barrier"], 1),
+                                 "epoch_errors": r["epoch_errors"]}
     out["barrier_full_occupancy"] = bar
     # the two-level barrier over a sweep of workgroup counts (1024 threads each)
     sweep = {}
@@ -363,7 +367,7 @@ def configs_leg(rt, reps=5, coop=False):
     return out
 
 
-def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5, apps=None):
+def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5, apps=None, coop=False):
     """Standalone persistent kernels on their configured workloads: BFS on
     ER 2^16 (avg degree 8), SSSP on RMAT scale 20 (ef 16, w in [1, 255]),
     color / MIS / PageRank on the 2048x2048 grid. One JSON line per app with
@@ -506,12 +510,15 @@ def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), r
         dg.close()
     if want("barrier"):
         # barrier round trip at full occupancy
-        for var, name in ((0, "flags"), (2, "xcd"), (1, "sense"), (3, "hip_cooperative_grid_sync")):
-            for wg in (1024, 256):
+        for var, name, wg in barrier_cases(coop):
+            try:
                 r = rt.barrier_bench(wg, 0, 200, var)
-                rec = {"app": "barrier", "variant": name, "wg_size": wg, "participating": r["participating"],
-                       "ns_per_barrier": round(r["ns_per_barrier"], 1), "epoch_errors": r["epoch_errors"]}
-                print(json.dumps(rec), flush=True)
+            except RuntimeError as e:  # WgmError: a wait ran out of its spin budget
+                print(json.dumps({"app": "barrier", "variant": name, "wg_size": wg, "error": str(e)}), flush=True)
+                continue
+            rec = {"app": "barrier", "variant": name, "wg_size": wg, "participating": r["participating"],
+                   "ns_per_barrier": round(r["ns_per_barrier"], 1), "epoch_errors": r["epoch_errors"]}
+            print(json.dumps(rec), flush=True)
     rt.close()
     return out
 
@@ -522,9 +529,9 @@ def main():
         shapes = tuple(tuple(int(x) for x in sh.split("x")) for sh in args.shapes.split(",")) if args.shapes else None
         apps = set(args.apps.split(",")) if args.apps else None
         if shapes:
-            suite_apps(shapes=shapes, reps=args.reps, apps=apps)
+            suite_apps(shapes=shapes, reps=args.reps, apps=apps, coop=args.coop_barrier)
         else:
-            suite_apps(reps=args.reps, apps=apps)
+            suite_apps(reps=args.reps, apps=apps, coop=args.coop_barrier)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
