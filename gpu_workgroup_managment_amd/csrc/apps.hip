@@ -42,8 +42,8 @@ __global__ void __launch_bounds__(1024) k_frontier(Ctl c, GraphDev g, FrontierDe
   extern __shared__ int lds[];
   Team t;
   if (!standalone_begin(c, t)) return;
-  run_team<SSSP, SSSP ? kEdgeSlots : 1>(g, s, nf, src, t.id, t.n, lds, iters_out,
-                                        [&]() { xcd_barrier<false>(c, t); });
+  run_team<SSSP, SSSP ? kEdgeSlots : 1, true>(g, s, nf, src, t.id, t.n, lds, iters_out,
+                                              [&]() { xcd_barrier<false>(c, t); });
   clock_end(c, t.id);
   standalone_retire(c);
 }

@@ -537,37 +537,67 @@ __device__ __forceinline__ void frontier_level(const GraphDev& g, const Frontier
 }
 
 // Second pass of a team's level over the heavy entries frontier_batch
-// deferred: for each in turn, every workgroup claims bd edges at a time until
-// its out-edges are done, so a hub's edges spread over the whole team. Runs
-// between two team barriers; the pushes go to the same next-level queue.
+// deferred. Their out-edges are taken as one list, bd entries at a time: the
+// workgroup scans the bd entries' degrees (as frontier_batch does), and every
+// workgroup of the team claims bd consecutive edges of that list per claim
+// (one counter per bd entries, hclaim[chunk], zeroed when the entries were
+// appended), so a hub's edges spread over the whole team and a workgroup
+// claims about (heavy edges / bd) / team size times per chunk, not once per
+// heavy entry. Runs between two team barriers; the pushes go to the same
+// next-level queue.
 template <bool SSSP, bool PAIRS>
 __device__ __forceinline__ void frontier_heavy(const GraphDev& g, const FrontierDev& s, int level, int* lds,
                                                uint32_t thr = 0xffffffffu, int32_t* far_q = nullptr,
                                                int* far_cnt = nullptr, uint32_t* in_far = nullptr) {
   const int bd = blockDim.x, tid = threadIdx.x;
+  int* s_end = lds;
+  int* s_beg = lds + bd;
+  uint32_t* s_du = (uint32_t*)(lds + 2 * bd);
+  int* s_wave = lds + 3 * bd;
+  int* s_claim = lds + 3 * bd + 17;
   int32_t* out_q = (level & 1) ? s.q0 : s.q1;
   uint32_t* out_qa = (level & 1) ? s.qa0 : s.qa1;
   int* out_cnt = &s.counts[((level + 1) % 3) * kLine];
   const uint32_t next = (uint32_t)level + 1u;
-  int* s_claim = lds + 3 * bd + 17;
   const int nh = __builtin_amdgcn_readfirstlane(ld_relaxed(&s.hcnt[(level % 3) * kLine]));
-  for (int h = 0; h < nh; h++) {
-    const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&s.hv[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const uint32_t du = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (int)__hip_atomic_load(&s.hdu[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const int beg = __builtin_amdgcn_readfirstlane(g.row_ptr[v]);
-    const int deg = __builtin_amdgcn_readfirstlane(g.row_ptr[v + 1]) - beg;
+  for (int h0 = 0; h0 < nh; h0 += bd) {
+    int beg = 0, deg = 0;
+    uint32_t du = 0;
+    if (h0 + tid < nh) {
+      const int v = __hip_atomic_load(&s.hv[h0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      du = __hip_atomic_load(&s.hdu[h0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      beg = g.row_ptr[v];
+      deg = g.row_ptr[v + 1] - beg;
+    }
+    int total;
+    const int excl = wg_exclusive_scan(deg, s_wave, &total);
+    s_end[tid] = excl + deg;
+    s_beg[tid] = beg - excl;
+    s_du[tid] = du;
+    int* claim = &s.hclaim[h0 / bd];
     while (true) {
-      if (tid == 0) *s_claim = __hip_atomic_fetch_add(&s.hclaim[h], bd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // the scan's stores above, or the last claim's reads of s_claim, come first
+      __syncthreads();
+      if (tid == 0) *s_claim = __hip_atomic_fetch_add(claim, bd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
       const int c0 = __builtin_amdgcn_readfirstlane(*s_claim);
-      __syncthreads();
-      if (c0 >= deg) break;
+      if (c0 >= total) break;
       const int e = c0 + tid;
-      relax_push<SSSP, false, true, PAIRS>(g, s, e < deg, beg + e, du, out_q, out_qa, out_cnt, next, thr, far_q,
+      int idx = 0;
+      uint32_t du_e = 0;
+      if (e < total) {
+        int lo = 0, hi = bd - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_end[mid] > e) hi = mid; else lo = mid + 1;
+        }
+        idx = s_beg[lo] + e;
+        du_e = s_du[lo];
+      }
+      relax_push<SSSP, false, true, PAIRS>(g, s, e < total, idx, du_e, out_q, out_qa, out_cnt, next, thr, far_q,
                                            far_cnt, in_far);
-      __syncthreads();  // the body ends in a barrier: the next claim by thread 0 stays one loop
     }
+    __syncthreads();  // every thread has read s_end before the next chunk's scan rewrites it
   }
 }
 

@@ -127,10 +127,12 @@ __device__ __forceinline__ void frontier_bottom_up(const GraphDev& g, const Fron
 // team's barrier) only has to drain, never to write back or invalidate an
 // L2. SSSP runs near-far bands of width nf.delta (0: frontier
 // Bellman-Ford), with the far counters and band minimum in device lines.
-// U: edges per thread per frontier pass (frontier_batch).
+// U: edges per thread per frontier pass (frontier_batch). HEAVY: teams whose
+// FrontierDev sets heavy > 0 (the standalone kernels) run the heavy-entry
+// pass; the batched kernel's teams (heavy = 0) are built without it.
 // Reference: drelax2 (src/lonestar/graph_apps/bfs/device/bfs.cl:214-302,
 // src/lonestar/graph_apps/sssp/device/sssp.cl:262-318).
-template <bool SSSP, int U = 1, class Barrier>
+template <bool SSSP, int U = 1, bool HEAVY = false, class Barrier>
 __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar& nf, int src, int rank, int k,
                          int* lds, int* iters_out, Barrier&& bar) {
   const bool near_far = SSSP && nf.delta > 0;
@@ -175,7 +177,7 @@ __device__ void run_team(const GraphDev& g, const FrontierDev& s, const NearFar&
       bar();
       // heavy entries deferred by the pass (read after the barrier: every
       // member sees the same count and takes the same number of barriers)
-      if (s.heavy > 0 && ld_relaxed(&s.hcnt[(level % 3) * kLine]) != 0) {
+      if (HEAVY && s.heavy > 0 && ld_relaxed(&s.hcnt[(level % 3) * kLine]) != 0) {
         frontier_heavy<SSSP, SSSP>(g, s, level, lds, thr, near_far ? nf.far(fi) : nullptr, &nf.cnt[fi * kLine],
                                    nf.flag);
         bar();
