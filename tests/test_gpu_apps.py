@@ -192,11 +192,39 @@ def _hub_chain(levels=6, leaves=(3000, 6000), seed=5, wmax=256):
     return G.from_edges(n, src, dst, wt)
 
 
+@pytest.mark.parametrize("wg", [64, 256])
+def test_frontier_heavy_list_of_several_chunks(rt, wg):
+    """A level of 300 heavy entries (1100-1500 edges each, above the 1024-edge
+    threshold of 64- and 256-thread workgroups) is more than one workgroup's
+    width of the heavy list: the pass scans it in several chunks, each with its
+    own claim counter. BFS levels and SSSP distances stay exact."""
+    rng = np.random.default_rng(11)
+    nhub, leaves = 300, 60000
+    n = 1 + nhub + leaves
+    degs = rng.integers(1100, 1501, nhub)
+    src = [np.zeros(nhub, np.int64)] + [np.full(d, 1 + h) for h, d in enumerate(degs)]
+    dst = [np.arange(1, 1 + nhub)] + [rng.integers(1 + nhub, n, d) for d in degs]
+    src.append(rng.integers(1 + nhub, n, 3 * leaves))
+    dst.append(rng.integers(1, n, 3 * leaves))
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    g = G.from_edges(n, src, dst, rng.integers(1, 5000, src.size).astype(np.int32))
+    assert (g.degrees() > 1024).sum() >= nhub
+    dg = rt.graph(g)
+    dist, st = dg.bfs(0, wg_size=wg, num_wgs=8)
+    ref, levels = oracle.bfs(g.row_ptr, g.col, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["iterations"] == levels
+    dist, st = dg.sssp(0, wg_size=wg, num_wgs=8, mode=L.SSSP_PUSH)
+    ref = oracle.sssp(g.row_ptr, g.col, g.wt, 0)
+    assert np.array_equal(dist, ref), _first_diff(dist, ref)
+    assert st["edges_scanned"] == int(g.degrees()[ref != oracle.INF].sum())
+
+
 @pytest.mark.parametrize("wg", [64, 256, 1024])
 def test_frontier_heavy_entries(rt, wg):
     """Entries of thousands of edges are deferred and their edges claimed in
-    pieces by every workgroup; BFS levels and SSSP distances stay exact, the
-    latter with weights large enough for many near-far bands."""
+    pieces of one list by every workgroup; BFS levels and SSSP distances stay
+    exact, the latter with weights large enough for many near-far bands."""
     g = _hub_chain()
     assert g.degrees().max() > 4 * 1024
     dg = rt.graph(g)
