@@ -26,11 +26,15 @@ namespace wgm {
 
 // Band widths of the standalone SSSP's near-far bands for weights in
 // [1, 255]: wide on graphs of out-degree <= 4 (long, narrow traversals: fewer
-// levels), narrow on the others (short, wide ones: less rework). Measured on
-// RMAT scale 20 (ef 16), 1024 x 256 workgroups: 8.24 ms at 1024, 7.21 at 64,
-// 6.32 at 16 (63 levels), 8.60 at 8.
+// levels), narrower on the others (short, wide ones: less rework). Measured on
+// RMAT scale 20 (ef 16), push direction, 1024 x 64 workgroups, four edges per
+// thread per pass and the heavy entries claimed as one list: 3.32 ms at 12
+// (78 levels), 3.09 at 16, 2.83 at 32, 2.76 at 48, 2.66 at 96, 2.65 at 128
+// (18 levels), 2.70 at 192 and 256, 2.67 at 512. With one edge per pass and
+// the heavy entries claimed one by one, 16 had been best (6.32 ms at 256
+// workgroups; 7.21 at 64, 8.24 at 1024): wide levels then cost more.
 constexpr uint32_t kStandaloneDelta = 1024;
-constexpr uint32_t kStandaloneDeltaHighDegree = 16;
+constexpr uint32_t kStandaloneDeltaHighDegree = 128;
 
 // BFS / SSSP over the whole grid as one team (run_team): state shared through
 // agent-scope atomics, so the per-level two-level barrier skips its L2
