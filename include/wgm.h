@@ -265,7 +265,8 @@ typedef struct wgm_run_stats {
  * wgm_sssp: Lonestar SSSP (src/lonestar/graph_apps/sssp/sssp.h:16-382,
  *   src/lonestar/graph_apps/sssp/device/standalone.cl), same distances: by
  *   default pull sweeps over the in-edges on graphs with an out-degree above
- *   4, and on the others push in near-far bands of width 1024 (lc->mode);
+ *   4, and on the others push in near-far bands of width 1024 (lc->mode;
+ *   a push on a graph of out-degree above 4 takes bands of width 128);
  *   st->edges_scanned counts the reached vertices' out-edges either way;
  * wgm_color: Pannotia color (src/pannotia/graph_apps/color/color.h:41-195,
  *   src/pannotia/graph_apps/color/device/standalone.cl:10-126);
