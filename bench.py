@@ -278,13 +278,13 @@ def configs_leg(rt, reps=5, coop=False):
     t0 = time.perf_counter()
     ref = oracle.bfs_threads(g.row_ptr, g.col, 0, threads)
     cpu = time.perf_counter() - t0
-    k, (res, st) = timed(lambda: dg.bfs(0, wg_size=1024, num_wgs=64))
+    k, (res, st) = timed(lambda: dg.bfs(0, wg_size=512, num_wgs=64))
     reached = ref != oracle.INF
     ms = int(g.degrees()[reached].sum())
     out["er_bfs"] = entry(k, 8 * ms + 20 * int(reached.sum()), np.array_equal(res, ref), cpu, {
         "workload": "ER n=2^16 avg degree 8, source 0", "levels": st["iterations"], "gteps": round(ms / (k * 1e6), 4),
         "us_per_level": round(st["device_ms"] * 1e3 / max(1, st["iterations"]), 2),
-        "launch": {"wg_size": 1024, "num_wgs": 64}, "cpu_threads": threads})
+        "launch": {"wg_size": 512, "num_wgs": 64}, "cpu_threads": threads})
     dg.close()
     # 2. SSSP, RMAT scale 20
     g = G.rmat_config()
@@ -367,7 +367,8 @@ def configs_leg(rt, reps=5, coop=False):
     return out
 
 
-def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64)), reps=5, apps=None, coop=False):
+def suite_apps(shapes=((1024, 0), (512, 0), (256, 0), (256, 128), (1024, 64), (512, 64)), reps=5, apps=None,
+               coop=False):
     """Standalone persistent kernels on their configured workloads: BFS on
     ER 2^16 (avg degree 8), SSSP on RMAT scale 20 (ef 16, w in [1, 255]),
     color / MIS / PageRank on the 2048x2048 grid. One JSON line per app with
